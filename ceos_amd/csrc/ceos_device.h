@@ -1,0 +1,313 @@
+// Device building blocks shared by the coCEOs kernels (gfx950, wave64).
+//
+//  * Proj<L>: sign-flip + fast Walsh-Hadamard projection of one vector per wave,
+//    F = 2^L coordinates, 3 rounds per repeat. Butterflies run in registers; the
+//    data is re-laid out between register phases through a per-wave LDS tile so
+//    that every butterfly stage is register-local. Stage order is ascending bit
+//    index (stride 1, 2, 4, ...), the order of the reference's fht_float
+//    (src/fht_avx.c:1012-1252), so the result is bit-identical to it.
+//  * radix_select: block-wide exact top-K over 64-bit keys (8-bit digits, early
+//    exit once the boundary digit bucket is taken whole).
+//  * bitonic_sort_desc: block-wide sort of a power-of-two LDS array.
+//  * Keys: fkey() maps an FP32 value to a uint32 whose unsigned order is the
+//    float order; composite keys put the value in the high word and ~id in the
+//    low word, so descending key order is (value desc, id asc) and keys are unique.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ceos {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t fkey(float v) {
+    uint32_t u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fkey_inv(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+}
+__device__ __forceinline__ uint64_t ckey(float v, uint32_t id) {
+    return ((uint64_t)fkey(v) << 32) | (uint64_t)(~id);
+}
+__device__ __forceinline__ uint32_t ckey_id(uint64_t k) { return ~(uint32_t)k; }
+__device__ __forceinline__ float ckey_val(uint64_t k) { return fkey_inv((uint32_t)(k >> 32)); }
+
+__device__ __forceinline__ float flip_if_zero_bit(float y, uint32_t bits, int r) {
+    // reference multiplies by (2*bit - 1): bit 0 -> -1 (sign flip), bit 1 -> +1
+    return __uint_as_float(__float_as_uint(y) ^ ((((~bits) >> r) & 1u) << 31));
+}
+
+// ----------------------------------------------------------------------------------
+// Projection of one vector by one wave.
+//
+// Register path (L >= 8): E = F/64 values per lane. In the phase whose register bits
+// start at bit b, lane `ln` register `r` holds element
+//     elem(ln, r, b) = (ln & (2^b - 1)) | (r << b) | ((ln >> b) << (b + e)),  e = L - 6.
+// Phases cover bits [0,e), [e,2e), ... and a last phase with b = L - e, whose layout
+// is element = r*64 + ln: lane-contiguous, so stores of the output are coalesced.
+// The first phase (b = 0) gives each lane E consecutive input coordinates.
+//
+// LDS path (L <= 7): the vector lives in LDS, one butterfly per lane per stage.
+// ----------------------------------------------------------------------------------
+template <int L, bool kRegs = (L >= 8)>
+struct Proj;
+
+template <int L>
+struct Proj<L, true> {
+    static constexpr int F = 1 << L;
+    static constexpr int e = L - 6;
+    static constexpr int E = 1 << e;
+    static constexpr int XE = E;        // input registers per lane
+    static constexpr int OE = E;        // output registers per lane (element r*64+lane)
+    static constexpr int kLdsWords = F + F / E;  // per-wave tile with one pad word per E
+
+    __device__ static __forceinline__ int pad(int i) { return i + (i >> e); }
+    __device__ static __forceinline__ int elem(int ln, int r, int b) {
+        return (ln & ((1 << b) - 1)) | (r << b) | ((ln >> b) << (b + e));
+    }
+
+    // x (d <= F floats) -> registers in the b = 0 layout (zero padded).
+    __device__ static void load_x(const float* __restrict__ x, int d, float (&xr)[XE], int ln) {
+        const int i0 = ln * E;
+        if (i0 + E <= d && ((reinterpret_cast<uintptr_t>(x + i0) & 15) == 0) && (E % 4 == 0)) {
+#pragma unroll
+            for (int r = 0; r < E; r += 4) {
+                float4 v = *reinterpret_cast<const float4*>(x + i0 + r);
+                xr[r] = v.x; xr[r + 1] = v.y; xr[r + 2] = v.z; xr[r + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < E; ++r) xr[r] = (i0 + r < d) ? x[i0 + r] : 0.0f;
+        }
+    }
+
+    template <int b, int t0, int t1>
+    __device__ static __forceinline__ void stages(float (&y)[E]) {
+#pragma unroll
+        for (int t = t0; t < t1; ++t) {
+            const int h = 1 << (t - b);
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                if (!(r & h)) {
+                    const float u = y[r], v = y[r + h];
+                    y[r] = u + v;
+                    y[r + h] = u - v;
+                }
+            }
+        }
+    }
+
+    __device__ static __forceinline__ void relayout(float (&y)[E], float* lds, int ln, int bf, int bt) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) lds[pad(elem(ln, r, bf))] = y[r];
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < E; ++r) y[r] = lds[pad(elem(ln, r, bt))];
+        wave_sync();
+    }
+
+    template <int covered, int bprev>
+    struct Phase {
+        static constexpr int b = (covered + e <= L) ? covered : (L - e);
+        __device__ static __forceinline__ void run(float (&y)[E], float* lds, int ln) {
+            if (covered > 0) relayout(y, lds, ln, bprev, b);
+            stages<b, covered, b + e>(y);
+            Phase<b + e, b>::run(y, lds, ln);
+        }
+    };
+    template <int bprev>
+    struct Phase<L, bprev> {
+        __device__ static __forceinline__ void run(float (&)[E], float*, int) {}
+    };
+
+    // Lane's E sign bits of round `i` (words of 32 bits, bit j of word w = coordinate 32w+j).
+    __device__ static __forceinline__ void sign_bits(const uint32_t* __restrict__ sg, int ln, uint32_t (&w)[2]) {
+        if (E >= 64) {
+            w[0] = sg[ln * 2]; w[1] = sg[ln * 2 + 1];
+        } else {
+            const int i0 = ln * E;
+            w[0] = (sg[i0 >> 5] >> (i0 & 31));
+            w[1] = 0;
+        }
+    }
+
+    // sg: 3 rounds x F/32 words for this repeat. Output y[r] = coordinate r*64 + ln.
+    __device__ static void run(const float (&xr)[XE], const uint32_t* __restrict__ sg, float* lds, int ln,
+                               float (&y)[OE]) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) y[r] = xr[r];
+#pragma unroll 1
+        for (int rd = 0; rd < 3; ++rd) {
+            if (rd > 0) relayout(y, lds, ln, L - e, 0);
+            uint32_t w[2];
+            sign_bits(sg + rd * (F / 32), ln, w);
+#pragma unroll
+            for (int r = 0; r < E; ++r) y[r] = flip_if_zero_bit(y[r], w[r >> 5], r & 31);
+            Phase<0, 0>::run(y, lds, ln);
+        }
+    }
+};
+
+template <int L>
+struct Proj<L, false> {
+    static constexpr int F = 1 << L;
+    static constexpr int OE = (F + 63) / 64;
+    static constexpr int XE = OE;
+    static constexpr int kLdsWords = F;
+
+    __device__ static void load_x(const float* __restrict__ x, int d, float (&xr)[XE], int ln) {
+#pragma unroll
+        for (int r = 0; r < XE; ++r) {
+            const int i = r * 64 + ln;
+            xr[r] = (i < d) ? x[i] : 0.0f;
+        }
+    }
+
+    __device__ static void run(const float (&xr)[XE], const uint32_t* __restrict__ sg, float* lds, int ln,
+                               float (&y)[OE]) {
+#pragma unroll
+        for (int r = 0; r < XE; ++r)
+            if (r * 64 + ln < F) lds[r * 64 + ln] = xr[r];
+        wave_sync();
+#pragma unroll 1
+        for (int rd = 0; rd < 3; ++rd) {
+            const uint32_t* s = sg + rd * ((F + 31) / 32);
+#pragma unroll
+            for (int r = 0; r < XE; ++r) {
+                const int i = r * 64 + ln;
+                if (i < F) lds[i] = flip_if_zero_bit(lds[i], s[i >> 5], i & 31);
+            }
+            wave_sync();
+#pragma unroll 1
+            for (int h = 1; h < F; h <<= 1) {
+                if (ln < F / 2) {
+                    const int i = ((ln / h) * 2 * h) + (ln % h);
+                    const float u = lds[i], v = lds[i + h];
+                    lds[i] = u + v;
+                    lds[i + h] = u - v;
+                }
+                wave_sync();
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < OE; ++r) {
+            const int i = r * 64 + ln;
+            y[r] = (i < F) ? lds[i] : 0.0f;
+        }
+        wave_sync();
+    }
+};
+
+// ----------------------------------------------------------------------------------
+// Block-wide helpers (blockDim.x == NT, NT a multiple of 64).
+// ----------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int ln) {
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o, 64);
+        if (ln >= o) incl += t;
+    }
+    return incl - v;
+}
+
+struct SelectShared {
+    uint32_t hist[256];
+    uint32_t bin, need, cnt, pad_;
+    uint32_t counter;
+};
+
+// Exact top-K (K >= 1) of n keys given by get(i). On return, key k is selected iff
+// (k >> shift) >= (prefix >> shift). With unique keys exactly min(K, n) keys are
+// selected; with repeated keys and min_shift > 0 the prefix holds the K-th largest
+// key's bits above min_shift. Must be called by all threads of the block.
+template <int NT, typename GetKey>
+__device__ void radix_select(int n, int K, GetKey get, SelectShared& sh, uint64_t& prefix_out, int& shift_out,
+                             int min_shift = 0) {
+    const int tid = threadIdx.x;
+    if (K >= n) {  // everything selected
+        prefix_out = 0;
+        shift_out = 63;
+        return;
+    }
+    uint64_t prefix = 0, mask = 0;
+    uint32_t need = (uint32_t)K;
+    int shift = 56;
+    for (;;) {
+        for (int b = tid; b < 256; b += NT) sh.hist[b] = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += NT) {
+            const uint64_t k = get(i);
+            if ((k & mask) == prefix) atomicAdd(&sh.hist[(uint32_t)(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            const int ln = tid;
+            uint32_t c[4], tot = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { c[j] = sh.hist[255 - 4 * ln - j]; tot += c[j]; }
+            uint32_t ex = wave_excl_scan(tot, ln);
+            if (ex < need && need <= ex + tot) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (need <= ex + c[j]) {
+                        sh.bin = 255 - 4 * ln - j;
+                        sh.need = need - ex;
+                        sh.cnt = c[j];
+                        break;
+                    }
+                    ex += c[j];
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t bin = sh.bin, nneed = sh.need, cnt = sh.cnt;
+        __syncthreads();
+        prefix |= (uint64_t)bin << shift;
+        mask |= (uint64_t)0xff << shift;
+        need = nneed;
+        if (cnt == need || shift <= min_shift) break;
+        shift -= 8;
+    }
+    prefix_out = prefix;
+    shift_out = shift;
+}
+
+__device__ __forceinline__ bool selected(uint64_t k, uint64_t prefix, int shift) {
+    return shift >= 63 ? true : ((k >> shift) >= (prefix >> shift));
+}
+
+// Descending bitonic sort of N (power of two) 64-bit keys in LDS.
+template <int NT>
+__device__ void bitonic_sort_desc(uint64_t* s, int N) {
+    const int tid = threadIdx.x;
+    for (int size = 2; size <= N; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = tid; t < (N >> 1); t += NT) {
+                const int i = 2 * t - (t & (stride - 1));
+                const int j = i + stride;
+                const bool desc = (i & size) == 0;
+                const uint64_t a = s[i], b = s[j];
+                if ((a < b) == desc) { s[i] = b; s[j] = a; }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__host__ __device__ __forceinline__ int next_pow2(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+}  // namespace ceos

@@ -1,0 +1,114 @@
+// Host-side launch interface of the coCEOs HIP kernels (internal to libceos_amd).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ceos {
+
+// Per-index constants shared by every kernel.
+struct IndexDims {
+    int d;        // features
+    int L;        // log2 of the FWHT size F
+    int D;        // projections kept per repeat (n_proj)
+    int R;        // repeats
+    int m;        // bucket depth (top_m): stride of a bucket list
+};
+
+// Bucket entry: x = point id, y = float bits of the stored value
+// (close lists: projection; far lists: -projection, src/CEOs.cpp:425-426).
+using Entry = uint2;
+
+// Row addressing for the stored points: logical id -> row slot. A static index uses
+// head = 0, cap = n. A stream index keeps a ring of `cap` rows whose logical id 0 is
+// at slot `head`; bucket ids are absolute and logical = abs - base.
+struct RowRing {
+    const float* X;
+    int64_t cap;
+    int64_t head;
+    uint32_t base;   // absolute id of logical id 0
+    uint32_t live;   // number of live rows
+};
+
+// ---------------- build ----------------
+// Projects rows row(k) = first + k*stride (k < nrows) for all R repeats.
+// col_major: out[col * ld_out + k]; else out[k * ld_out + col], col = r*D + j.
+hipError_t launch_project_rows(const IndexDims& dm, const float* X, int64_t first, int64_t stride,
+                               int64_t nrows, const uint32_t* signs, float* out, int64_t ld_out,
+                               bool col_major, hipStream_t st);
+
+// Per list (2*DR lists: list 2c = close, 2c+1 = far) threshold = rank-th largest of the
+// column's sample values (far: of -value). sample is column-major [DR][S].
+hipError_t launch_sample_thresholds(const float* sample, int64_t S, int DR, int rank, float* thr,
+                                    hipStream_t st);
+
+// Streams all n points, projects them and appends (id, value) to list L when
+// value >= thr[L] (close) / -value >= thr[L] (far). Lists with active[L] == 0 are
+// skipped (active may be null). Writes only while pos < cap[L]; count[L] counts all.
+hipError_t launch_scan_append(const IndexDims& dm, const float* X, int64_t n, const uint32_t* signs,
+                              const float* thr, const uint8_t* active, const int64_t* base,
+                              const int32_t* cap, int32_t* count, Entry* ent, hipStream_t st);
+
+// Per list: top-m of ent[base[L] .. base[L]+min(count[L],cap[L])) by (value desc, id asc),
+// written sorted into bkt[L*m ...]; size[L] = min(m, available).
+hipError_t launch_list_select(int nlists, int m, const int64_t* base, const int32_t* cap,
+                              const int32_t* count, const Entry* ent, Entry* bkt, int32_t* size,
+                              hipStream_t st);
+
+// Stream update: per list, top-m of {old entries whose logical id (abs - new_base) is
+// below live} U {new points abs ids first_abs + j with values from newproj (col-major
+// [DR][n_new])}; ties prefer the smaller logical id.
+hipError_t launch_stream_merge(int DR, int m, Entry* bkt, int32_t* size, const float* newproj,
+                               int64_t n_new, uint32_t first_abs, uint32_t new_base, uint32_t live,
+                               hipStream_t st);
+
+// Removes entries whose row is flagged in `dead` (bitmap over ring slots).
+hipError_t launch_bucket_purge(int nlists, int m, Entry* bkt, int32_t* size, const uint32_t* dead,
+                               RowRing ring, hipStream_t st);
+
+// ---------------- query ----------------
+struct QueryBatch {
+    const float* Q;          // nq x d
+    int nq;
+    const int32_t* k;        // per-query arrays, or null to use k0/s0/c0
+    const int32_t* s;
+    const int32_t* c;
+    int k0, s0, c0;
+    int mp;                  // n_probed_points
+    const int64_t* out_off;  // per-query output offset (null: q * k0)
+    int32_t* out_id;
+    float* out_ip;
+};
+
+struct QueryConfig {
+    int grid;          // persistent workgroups
+    int table_log2;    // LDS hash table slots = 2^table_log2
+    int lds_bytes;     // dynamic LDS per workgroup
+    int s_max, c_max, k_max;
+    int64_t cand_stride;  // scratch entries per workgroup
+};
+
+struct QueryScratch {
+    uint64_t* cand;    // grid * cand_stride
+    int32_t* work;     // work counter (1 int)
+    int32_t* status;   // error flags (1 int): bit0 = hash table overflow
+};
+
+// Fused coCEOs search (projection, probe selection, score accumulation, candidate
+// selection, rerank, top-k) over a persistent grid.
+hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, const Entry* bkt,
+                                const int32_t* bsize, RowRing rows, const QueryBatch& qb,
+                                const QueryConfig& cfg, const QueryScratch& scr, hipStream_t st);
+
+int coceos_search_lds_bytes(const IndexDims& dm, int table_log2, int s_max, int c_max, int k_max);
+
+// CEOs-Est (dense) search over column-major projections P [DR][n].
+hipError_t launch_est_search(const IndexDims& dm, const uint32_t* signs, const float* P, int64_t n,
+                             const float* X, const QueryBatch& qb, int grid, uint64_t* scratch,
+                             int64_t scratch_stride, int32_t* work, hipStream_t st);
+
+int est_search_lds_bytes(const IndexDims& dm, int s_max, int c_max, int k_max);
+
+hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
+
+}  // namespace ceos

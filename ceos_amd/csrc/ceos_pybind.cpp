@@ -1,0 +1,354 @@
+// pybind11 module `CEOs`: the reference's Python surface (python/python_wrapper.cpp:13-75)
+// over the C ABI of libceos_amd (include/ceos_amd.h).
+//
+// Same class, method, argument and field names and defaults. Differences, all
+// stricter: invalid parameters raise ValueError instead of exit(1)
+// (src/CEOs.cpp:452-461); device failures raise RuntimeError; the GIL is released
+// while the device works. build_CEOs_Hash / search_CEOs_Hash / hash_search are not
+// provided (CEOs-Hash is outside this library's scope) and raise NotImplementedError.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+
+#include <chrono>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/ceos_amd.h"
+
+namespace py = pybind11;
+
+namespace {
+
+using FArr = py::array_t<float, py::array::c_style | py::array::forcecast>;
+using IArr = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
+
+void check(int rc) {
+    if (rc == CEOS_OK) return;
+    const std::string msg = ceos_last_error();
+    if (rc == CEOS_EINVAL) throw py::value_error(msg);
+    if (rc == CEOS_ENOMEM) throw std::runtime_error("device memory: " + msg);
+    throw std::runtime_error(msg);
+}
+
+void require_2d(const FArr& a, const char* what) {
+    if (a.ndim() != 2) throw py::value_error(std::string(what) + " must be a 2-D array (rows x n_features)");
+}
+
+[[noreturn]] void not_provided(const char* what) {
+    PyErr_SetString(PyExc_NotImplementedError, what);
+    throw py::error_already_set();
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+py::tuple result(int nq, int k, py::array_t<int32_t>& ids, py::array_t<float>& ips) {
+    (void)nq; (void)k;
+    return py::make_tuple(ids, ips);
+}
+
+class PyCEOs {
+   public:
+    PyCEOs(int n_points, int n_features) : n_(n_points), d_(n_features) { check(ceos_create(n_points, n_features, &h_)); }
+    ~PyCEOs() { ceos_destroy(h_); }
+
+    void setIndexParam(int n_proj, int n_repeats, int top_m, int n_threads, int random_seed) {
+        check(ceos_set_index_param(h_, n_proj, n_repeats, top_m, n_threads, random_seed));
+    }
+
+    int get_s() const { int s, mp, c; check(ceos_get_query_param(h_, &s, &mp, &c)); return s; }
+    int get_mp() const { int s, mp, c; check(ceos_get_query_param(h_, &s, &mp, &c)); return mp; }
+    int get_c() const { int s, mp, c; check(ceos_get_query_param(h_, &s, &mp, &c)); return c; }
+    void set_s(int v) { check(ceos_set_query_param(h_, v, get_mp(), get_c())); }
+    void set_mp(int v) { check(ceos_set_query_param(h_, get_s(), v, get_c())); }
+    void set_c(int v) { check(ceos_set_query_param(h_, get_s(), get_mp(), v)); }
+    bool get_centering() const { int v; check(ceos_get_centering(h_, &v)); return v != 0; }
+    void set_centering(bool v) { check(ceos_set_centering(h_, v ? 1 : 0)); }
+    void set_threads(int t) { check(ceos_set_threads(h_, t)); }
+    void clear() { check(ceos_clear(h_)); }
+
+    void print_params(const char* title) const {
+        int F, D, R, m, seed;
+        check(ceos_get_dims(h_, &F, &D, &R, &m, &seed));
+        std::printf("%s\nn_points: %d\nn_features: %d\nn_repeats: %d\nn_proj: %d\ntop_m: %d\nfhtDim: %d\n", title, n_,
+                    d_, R, D, m, F);
+    }
+
+    void build_coCEOs_Est(FArr X) {
+        require_2d(X, "dataset");
+        print_params("Building coCEOs-Estimate index...");
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_build_coceos(h_, X.data(), X.shape(0), (int)X.shape(1));
+        }
+        check(rc);
+        std::printf("Constructing time (in seconds): %.3f\n", ms_since(t0) / 1000.0);
+        std::fflush(stdout);
+    }
+
+    py::tuple search_coCEOs_Est(FArr Q, int k, bool verbose) {
+        require_2d(Q, "queries");
+        const int nq = (int)Q.shape(0);
+        if (k < 0) throw py::value_error("n_neighbors must be >= 0");
+        py::array_t<int32_t> ids({nq, k});
+        py::array_t<float> ips({nq, k});
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_search_coceos(h_, Q.data(), nq, (int)Q.shape(1), k, ids.mutable_data(), ips.mutable_data());
+        }
+        check(rc);
+        if (verbose) {
+            std::printf("n_queries: %d\nn_probedVectors: %d\nn_probedPoints: %d\nn_cand: %d\nQuerying time: %.3f ms\n", nq,
+                        get_s(), get_mp(), get_c(), ms_since(t0));
+            std::fflush(stdout);
+        }
+        return result(nq, k, ids, ips);
+    }
+
+    // Batched form: per-query k, n_probed_vectors and n_cand. Returns (ids, ips, offsets)
+    // with query q's results at offsets[q] .. offsets[q+1]-1.
+    py::tuple search_batched(FArr Q, IArr k, IArr s, IArr c) {
+        require_2d(Q, "queries");
+        const int nq = (int)Q.shape(0);
+        if (k.size() != nq || s.size() != nq || c.size() != nq)
+            throw py::value_error("k, n_probed_vectors and n_cand arrays must have one entry per query");
+        py::array_t<int64_t> off(nq + 1);
+        int64_t tot = 0;
+        auto o = off.mutable_unchecked<1>();
+        for (int q = 0; q < nq; ++q) { o(q) = tot; tot += std::max<int32_t>(0, k.data()[q]); }
+        o(nq) = tot;
+        py::array_t<int32_t> ids(tot);
+        py::array_t<float> ips(tot);
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_search_batched(h_, Q.data(), nq, (int)Q.shape(1), k.data(), s.data(), c.data(),
+                                     ids.mutable_data(), ips.mutable_data());
+        }
+        check(rc);
+        return py::make_tuple(ids, ips, off);
+    }
+
+    void build(FArr X) {
+        require_2d(X, "dataset");
+        print_params("Building CEOs index...");
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_build_est(h_, X.data(), X.shape(0), (int)X.shape(1));
+        }
+        check(rc);
+        std::printf("Index construction time (in seconds): %.3f\n", ms_since(t0) / 1000.0);
+        std::fflush(stdout);
+    }
+
+    py::tuple search(FArr Q, int k, bool verbose) {
+        require_2d(Q, "queries");
+        const int nq = (int)Q.shape(0);
+        if (k < 0) throw py::value_error("n_neighbors must be >= 0");
+        py::array_t<int32_t> ids({nq, k});
+        py::array_t<float> ips({nq, k});
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_search_est(h_, Q.data(), nq, (int)Q.shape(1), k, ids.mutable_data(), ips.mutable_data());
+        }
+        check(rc);
+        if (verbose) {
+            std::printf("n_queries: %d\nn_probed_vectors: %d\nn_cand: %d\nQuerying time: %.3f ms\n", nq, get_s(),
+                        get_c(), ms_since(t0));
+            std::fflush(stdout);
+        }
+        return result(nq, k, ids, ips);
+    }
+
+    // Parity helpers (not in the reference surface).
+    py::tuple get_buckets() const {
+        int F, D, R, m, seed;
+        check(ceos_get_dims(h_, &F, &D, &R, &m, &seed));
+        const int DR = D * R;
+        py::array_t<int32_t> cid({DR, m}), fid({DR, m});
+        py::array_t<float> cv({DR, m}), fv({DR, m});
+        check(ceos_get_buckets(h_, cid.mutable_data(), cv.mutable_data(), fid.mutable_data(), fv.mutable_data()));
+        return py::make_tuple(cid, cv, fid, fv);
+    }
+    py::array_t<uint8_t> get_sign_bits() const {
+        int F, D, R, m, seed;
+        check(ceos_get_dims(h_, &F, &D, &R, &m, &seed));
+        py::array_t<uint8_t> out((size_t)F * 3 * R);
+        check(ceos_get_sign_bits(h_, out.mutable_data()));
+        return out;
+    }
+    py::array_t<float> project(FArr rows) {
+        require_2d(rows, "rows");
+        int F, D, R, m, seed;
+        check(ceos_get_dims(h_, &F, &D, &R, &m, &seed));
+        py::array_t<float> out({(int64_t)rows.shape(0), (int64_t)D * R});
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_project(h_, rows.data(), rows.shape(0), (int)rows.shape(1), out.mutable_data());
+        }
+        check(rc);
+        return out;
+    }
+    int fht_dim() const { int F, D, R, m, seed; check(ceos_get_dims(h_, &F, &D, &R, &m, &seed)); return F; }
+    void set_search_config(int wg_per_cu, int table_log2) { check(ceos_set_search_config(h_, wg_per_cu, table_log2)); }
+    py::tuple get_search_config() const {
+        int wg, t, lds;
+        check(ceos_get_search_config(h_, &wg, &t, &lds));
+        return py::make_tuple(wg, t, lds);
+    }
+    uintptr_t handle() const { return reinterpret_cast<uintptr_t>(h_); }
+
+   private:
+    ceos_index* h_ = nullptr;
+    int n_, d_;
+};
+
+class PyStream {
+   public:
+    explicit PyStream(int n_features) : d_(n_features) { check(ceos_stream_create(n_features, &h_)); }
+    ~PyStream() { ceos_stream_destroy(h_); }
+
+    void setIndexParam(int n_proj, int n_repeats, int top_points, int n_threads, int random_seed) {
+        check(ceos_stream_set_index_param(h_, n_proj, n_repeats, top_points, n_threads, random_seed));
+    }
+    int get_s() const { int s, mp, c; check(ceos_stream_get_query_param(h_, &s, &mp, &c)); return s; }
+    int get_mp() const { int s, mp, c; check(ceos_stream_get_query_param(h_, &s, &mp, &c)); return mp; }
+    int get_c() const { int s, mp, c; check(ceos_stream_get_query_param(h_, &s, &mp, &c)); return c; }
+    void set_s(int v) { check(ceos_stream_set_query_param(h_, v, get_mp(), get_c())); }
+    void set_mp(int v) { check(ceos_stream_set_query_param(h_, get_s(), v, get_c())); }
+    void set_c(int v) { check(ceos_stream_set_query_param(h_, get_s(), get_mp(), v)); }
+    void set_threads(int t) { check(ceos_stream_set_threads(h_, t)); }
+    void clear() { check(ceos_stream_clear(h_)); }
+
+    void build(FArr X) {
+        require_2d(X, "dataset");
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_stream_build(h_, X.data(), X.shape(0), (int)X.shape(1));
+        }
+        check(rc);
+        std::printf("n_features: %d\nn_points: %lld\nConstructing time (in seconds): %.3f\n", d_,
+                    (long long)X.shape(0), ms_since(t0) / 1000.0);
+        std::fflush(stdout);
+    }
+    void update(FArr newX, int n_del) {
+        require_2d(newX, "new_dataset");
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_stream_update(h_, newX.data(), newX.shape(0), (int)newX.shape(1), n_del);
+        }
+        check(rc);
+    }
+    void delete_ids(IArr ids) {
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_stream_delete_ids(h_, ids.data(), ids.size());
+        }
+        check(rc);
+    }
+    py::tuple estimate_search(FArr Q, int k, bool verbose) {
+        require_2d(Q, "queries");
+        const int nq = (int)Q.shape(0);
+        if (k < 0) throw py::value_error("n_neighbors must be >= 0");
+        py::array_t<int32_t> ids({nq, k});
+        py::array_t<float> ips({nq, k});
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_stream_estimate_search(h_, Q.data(), nq, (int)Q.shape(1), k, ids.mutable_data(),
+                                             ips.mutable_data());
+        }
+        check(rc);
+        if (verbose) {
+            std::printf("n_queries: %d\nn_probedVectors: %d\nn_cand: %d\nQuerying time: %.3f ms\n", nq, get_s(), get_c(),
+                        ms_since(t0));
+            std::fflush(stdout);
+        }
+        return py::make_tuple(ids, ips);
+    }
+    int64_t size() const { int64_t n; check(ceos_stream_size(h_, &n)); return n; }
+    py::tuple get_buckets(int DR, int m) const {
+        py::array_t<int32_t> cid({DR, m}), fid({DR, m}), cs(DR), fs(DR);
+        py::array_t<float> cv({DR, m}), fv({DR, m});
+        check(ceos_stream_get_buckets(h_, cid.mutable_data(), cv.mutable_data(), fid.mutable_data(),
+                                      fv.mutable_data(), cs.mutable_data(), fs.mutable_data()));
+        return py::make_tuple(cid, cv, fid, fv, cs, fs);
+    }
+
+   private:
+    ceos_stream* h_ = nullptr;
+    int d_;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(CEOs, m) {
+    m.doc() = "MI355X-native coCEOs (HIP, gfx950) with the reference CEOs module's Python API";
+
+    py::class_<PyCEOs>(m, "CEOs")
+        .def(py::init<int, int>(), py::arg("n_points"), py::arg("n_features"),
+             "Constructor for CEOs.\nn_points: number of points.\nn_features: number of dimensions.")
+        .def("setIndexParam", &PyCEOs::setIndexParam, py::arg("n_proj"), py::arg("n_repeats") = 1,
+             py::arg("top_m") = 10, py::arg("n_threads") = -1, py::arg("random_seed") = -1)
+        .def_property("n_probed_vectors", &PyCEOs::get_s, &PyCEOs::set_s)
+        .def_property("n_probed_points", &PyCEOs::get_mp, &PyCEOs::set_mp)
+        .def_property("n_cand", &PyCEOs::get_c, &PyCEOs::set_c)
+        .def_property("centering", &PyCEOs::get_centering, &PyCEOs::set_centering)
+        .def("set_threads", &PyCEOs::set_threads, py::arg("n_threads"))
+        .def("clear", &PyCEOs::clear)
+        .def("build", &PyCEOs::build, py::arg("dataset"))
+        .def("search", &PyCEOs::search, py::arg("queries"), py::arg("n_neighbors"), py::arg("verbose") = false)
+        .def("build_coCEOs_Est", &PyCEOs::build_coCEOs_Est, py::arg("dataset"))
+        .def("search_coCEOs_Est", &PyCEOs::search_coCEOs_Est, py::arg("queries"), py::arg("n_neighbors"),
+             py::arg("verbose") = false)
+        .def("build_CEOs_Hash", [](PyCEOs&, py::object) { not_provided("CEOs-Hash is not provided by this library"); },
+             py::arg("dataset"))
+        .def("search_CEOs_Hash",
+             [](PyCEOs&, py::object, int, bool) -> py::tuple { not_provided("CEOs-Hash is not provided by this library"); },
+             py::arg("queries"), py::arg("n_neighbors"), py::arg("verbose") = false)
+        .def("search_batched", &PyCEOs::search_batched, py::arg("queries"), py::arg("n_neighbors"),
+             py::arg("n_probed_vectors"), py::arg("n_cand"))
+        .def("get_buckets", &PyCEOs::get_buckets)
+        .def("get_sign_bits", &PyCEOs::get_sign_bits)
+        .def("project", &PyCEOs::project, py::arg("rows"))
+        .def_property_readonly("fhtDim", &PyCEOs::fht_dim)
+        .def("set_search_config", &PyCEOs::set_search_config, py::arg("wg_per_cu") = 0, py::arg("table_log2") = 0)
+        .def("get_search_config", &PyCEOs::get_search_config)
+        .def_property_readonly("_handle", &PyCEOs::handle);
+
+    py::class_<PyStream>(m, "streamCEOs")
+        .def(py::init<int>(), py::arg("n_features"))
+        .def("setIndexParam", &PyStream::setIndexParam, py::arg("n_proj"), py::arg("n_repeats") = 1,
+             py::arg("top_points") = 1, py::arg("n_threads") = -1, py::arg("random_seed") = -1)
+        .def_property("n_probed_vectors", &PyStream::get_s, &PyStream::set_s)
+        .def_property("n_probed_points", &PyStream::get_mp, &PyStream::set_mp)
+        .def_property("n_cand", &PyStream::get_c, &PyStream::set_c)
+        .def("set_threads", &PyStream::set_threads, py::arg("n_threads"))
+        .def("clear", &PyStream::clear)
+        .def("build", &PyStream::build, py::arg("dataset"))
+        .def("update", &PyStream::update, py::arg("new_dataset"), py::arg("n_delPoints") = 0)
+        .def("estimate_search", &PyStream::estimate_search, py::arg("queries"), py::arg("n_neighbors"),
+             py::arg("verbose") = false)
+        .def("hash_search",
+             [](PyStream&, py::object, int, bool) -> py::tuple { not_provided("streamCEOs.hash_search is not provided by this library"); },
+             py::arg("queries"), py::arg("n_neighbors"), py::arg("verbose") = false)
+        .def("delete_ids", &PyStream::delete_ids, py::arg("ids"))
+        .def("size", &PyStream::size)
+        .def("get_buckets", &PyStream::get_buckets, py::arg("n_lists"), py::arg("top_m"));
+}

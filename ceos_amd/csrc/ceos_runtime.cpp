@@ -1,0 +1,934 @@
+// Host runtime of libceos_amd: device-resident coCEOs / CEOs-Est / streamCEOs indexes
+// behind the C ABI declared in include/ceos_amd.h.
+//
+// Each index owns one HIP stream, its points (HBM copy of X), the sign diagonals,
+// the bucket lists and the query scratch. Calls on one index are serialised by a
+// mutex; different indexes run independently.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ceos_amd.h"
+#include "ceos_kernels.h"
+
+using ceos::Entry;
+using ceos::IndexDims;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct ArgError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct DevError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct MemError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct StateError : std::runtime_error { using std::runtime_error::runtime_error; };
+
+#define HIP_CHECK(expr)                                                                         \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw DevError(std::string(#expr) + " failed: " + hipGetErrorString(e_));           \
+    } while (0)
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_err.clear();
+        return CEOS_OK;
+    } catch (const ArgError& e) {
+        g_err = e.what();
+        return CEOS_EINVAL;
+    } catch (const MemError& e) {
+        g_err = e.what();
+        return CEOS_ENOMEM;
+    } catch (const StateError& e) {
+        g_err = e.what();
+        return CEOS_ESTATE;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return CEOS_EDEVICE;
+    }
+}
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        if (hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            throw MemError("device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void ensure(size_t count) {
+        if (count > n) alloc(count);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// src/CEOs.h:76-79
+int fht_dim(int n_proj, int d) {
+    if (n_proj < d) return 1 << (int)std::ceil(std::log2((double)d));
+    return 1 << (int)std::ceil(std::log2((double)n_proj));
+}
+
+// Sign bits of src/Utilities.cpp:21-41: std::default_random_engine (minstd_rand0,
+// x <- 16807 x mod (2^31-1)) drawn through libstdc++'s uniform_int_distribution(0,1),
+// which rejects x-1 >= 2147483644 and returns (x-1) / 1073741822.
+std::vector<uint8_t> sign_bits(int nbits, unsigned seed) {
+    uint64_t x = seed % 2147483647u;
+    if (x == 0) x = 1;
+    std::vector<uint8_t> bits(nbits);
+    for (int i = 0; i < nbits; ++i) {
+        uint64_t r;
+        do {
+            x = (x * 16807u) % 2147483647u;
+            r = x - 1;
+        } while (r >= 2147483644u);
+        bits[i] = (uint8_t)(r / 1073741822u);
+    }
+    return bits;
+}
+
+int device_cus() {
+    static int cus = [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+        return v;
+    }();
+    if (cus <= 0) throw DevError("no HIP device available");
+    return cus;
+}
+
+constexpr int kMaxLdsBytes = 160 * 1024 - 2048;  // minus the kernels' static LDS
+constexpr int kMaxTopM = 8192;
+constexpr int kMaxProbes = 4096;
+constexpr int kMaxK = 4096;
+
+// Device state shared by CEOs and streamCEOs.
+struct Core {
+    int d = 0;
+    int D = 256, R = 1, m = 100, F = 0, L = 0;
+    int seed = -1;
+    unsigned seed_used = 0;
+    int threads = 1;
+    int s = 10, mp = 10, c = 10;  // n_probed_vectors, n_probed_points, n_cand
+    bool centering = false;
+    bool params_set = false;
+    bool built = false;
+    hipStream_t st_ = nullptr;
+
+    std::vector<uint8_t> bits;
+    DBuf<uint32_t> signs;
+    DBuf<Entry> bkt;
+    DBuf<int32_t> bsize;
+    // stored points: ring of `cap` rows, logical id 0 at slot `head`, absolute id `base`
+    DBuf<float> X;
+    int64_t cap = 0, head = 0;
+    uint32_t base = 0, live = 0;
+    DBuf<uint32_t> dead;  // stream random-delete bitmap over ring slots
+
+    // query scratch
+    DBuf<uint64_t> cand;
+    DBuf<int32_t> work, status;
+    int wg_per_cu = 0, table_log2 = 0;
+    int last_lds = 0, last_tlog2 = 0, last_wg = 0;
+
+    // CEOs-Est projections [DR][n]
+    DBuf<float> estP;
+    int64_t est_n = 0;
+
+    std::mutex mu;
+
+    Core() = default;
+    ~Core() {
+        if (st_) (void)hipStreamDestroy(st_);
+    }
+    // The stream is created on first device use, so handles can be made and
+    // configured on a host without a GPU.
+    hipStream_t st() {
+        if (!st_) HIP_CHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+        return st_;
+    }
+
+    int DR() const { return D * R; }
+    IndexDims dims() const { return IndexDims{d, L, D, R, m}; }
+    hipStream_t stream(void* s_) { return s_ ? reinterpret_cast<hipStream_t>(s_) : st(); }
+
+    void set_index_param(int n_proj, int n_repeats, int top_m, int n_threads, int seed_) {
+        if (n_proj <= 0 || n_repeats <= 0 || top_m <= 0)
+            throw ArgError("n_proj, n_repeats and top_m must be positive");
+        const int f = fht_dim(n_proj, d);
+        if (f > 4096) throw ArgError("fhtDim " + std::to_string(f) + " exceeds 4096 (n_proj and n_features <= 4096)");
+        if (top_m > kMaxTopM) throw ArgError("top_m above " + std::to_string(kMaxTopM) + " is not supported");
+        D = n_proj; R = n_repeats; m = top_m; seed = seed_;
+        threads = n_threads;
+        F = f;
+        L = 0;
+        while ((1 << L) < F) ++L;
+        params_set = true;
+        built = false;
+    }
+
+    void clear() {
+        bkt.release(); bsize.release(); X.release(); signs.release(); dead.release();
+        cand.release(); work.release(); status.release(); estP.release();
+        bits.clear();
+        cap = head = 0; base = live = 0; est_n = 0;
+        built = false;
+    }
+
+    void make_signs() {
+        seed_used = seed >= 0 ? (unsigned)seed
+                              : (unsigned)std::chrono::system_clock::now().time_since_epoch().count();
+        bits = sign_bits(F * 3 * R, seed_used);
+        const int W = (F + 31) / 32;
+        std::vector<uint32_t> words((size_t)W * 3 * R, 0u);
+        for (int r = 0; r < R; ++r)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < F; ++j)
+                    if (bits[(size_t)r * 3 * F + (size_t)i * F + j])
+                        words[(size_t)(r * 3 + i) * W + j / 32] |= 1u << (j % 32);
+        signs.alloc(words.size());
+        HIP_CHECK(hipMemcpyAsync(signs.p, words.data(), words.size() * 4, hipMemcpyHostToDevice, st()));
+        HIP_CHECK(hipStreamSynchronize(st()));
+    }
+
+    // Copies n rows (host or device pointer) into a fresh ring starting at slot 0.
+    void load_points(const float* src, int64_t n, int64_t capacity, bool src_device, hipStream_t s_) {
+        X.alloc((size_t)capacity * d);
+        cap = capacity; head = 0; base = 0; live = (uint32_t)n;
+        HIP_CHECK(hipMemcpyAsync(X.p, src, (size_t)n * d * 4,
+                                 src_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s_));
+    }
+
+    // coCEOs buckets over rows 0..n-1 of X (ring slots 0..n-1): see ceos_build.hip.
+    void build_buckets(int64_t n, hipStream_t s_) {
+        const int DR_ = DR();
+        const int nl = 2 * DR_;
+        const IndexDims dm = dims();
+        if (m > n) throw ArgError("top_m must not exceed the number of points");
+
+        int64_t S_target = std::max<int64_t>(65536, (64 * n + m - 1) / m);
+        S_target = std::min<int64_t>(S_target, (int64_t(4) << 30) / (4LL * DR_));
+        S_target = std::max<int64_t>(S_target, 1);
+        const int64_t stride = std::max<int64_t>(1, n / S_target);
+        const int64_t S = (n + stride - 1) / stride;
+        const bool exact = stride == 1;
+        const int rank = exact ? m : (int)std::min<int64_t>(S, (2 * (int64_t)m * S + n - 1) / n + 8);
+
+        DBuf<float> thr(nl);
+        {
+            DBuf<float> sample((size_t)DR_ * S);
+            HIP_CHECK(ceos::launch_project_rows(dm, X.p, 0, stride, S, signs.p, sample.p, S, true, s_));
+            HIP_CHECK(ceos::launch_sample_thresholds(sample.p, S, DR_, rank, thr.p, s_));
+            HIP_CHECK(hipStreamSynchronize(s_));
+        }
+        std::vector<int64_t> hbase(nl);
+        std::vector<int32_t> hcap(nl, (int32_t)std::min<int64_t>(n, 4LL * m + 256));
+        std::vector<int32_t> hcount(nl);
+        std::vector<float> hthr;
+        DBuf<int64_t> dbase(nl);
+        DBuf<int32_t> dcap(nl), dcount(nl);
+        DBuf<Entry> ent;
+        for (int iter = 0;; ++iter) {
+            int64_t total = 0;
+            for (int l = 0; l < nl; ++l) { hbase[l] = total; total += hcap[l]; }
+            ent.alloc((size_t)total);
+            HIP_CHECK(hipMemcpyAsync(dbase.p, hbase.data(), nl * 8, hipMemcpyHostToDevice, s_));
+            HIP_CHECK(hipMemcpyAsync(dcap.p, hcap.data(), nl * 4, hipMemcpyHostToDevice, s_));
+            HIP_CHECK(ceos::launch_fill_i32(dcount.p, 0, nl, s_));
+            HIP_CHECK(ceos::launch_scan_append(dm, X.p, n, signs.p, thr.p, nullptr, dbase.p, dcap.p, dcount.p,
+                                               ent.p, s_));
+            HIP_CHECK(hipMemcpyAsync(hcount.data(), dcount.p, nl * 4, hipMemcpyDeviceToHost, s_));
+            HIP_CHECK(hipStreamSynchronize(s_));
+            bool redo = false;
+            for (int l = 0; l < nl; ++l) {
+                if (hcount[l] < m) {
+                    if (hthr.empty()) {
+                        hthr.resize(nl);
+                        HIP_CHECK(hipMemcpy(hthr.data(), thr.p, nl * 4, hipMemcpyDeviceToHost));
+                    }
+                    hthr[l] = -INFINITY;  // accept every point for this list
+                    hcap[l] = (int32_t)n;
+                    redo = true;
+                } else if (hcount[l] > hcap[l]) {
+                    hcap[l] = hcount[l];
+                    redo = true;
+                }
+            }
+            if (!redo) break;
+            if (iter >= 2) throw DevError("bucket construction did not converge");
+            if (!hthr.empty()) HIP_CHECK(hipMemcpy(thr.p, hthr.data(), nl * 4, hipMemcpyHostToDevice));
+        }
+        bkt.alloc((size_t)nl * m);
+        bsize.alloc(nl);
+        HIP_CHECK(ceos::launch_list_select(nl, m, dbase.p, dcap.p, dcount.p, ent.p, bkt.p, bsize.p, s_));
+        HIP_CHECK(hipStreamSynchronize(s_));
+    }
+
+    // Picks the LDS score-tile size and grid for a search with the given bounds.
+    ceos::QueryConfig search_config(int s_max, int c_max, int k_max) {
+        const IndexDims dm = dims();
+        const int wg = wg_per_cu > 0 ? wg_per_cu : 2;
+        const int budget = std::min(kMaxLdsBytes, (160 * 1024) / wg - 2048);
+        int t = 0;
+        if (table_log2 > 0) {
+            t = table_log2;
+        } else {
+            for (int cand_t = 16; cand_t >= 8; --cand_t)
+                if (ceos::coceos_search_lds_bytes(dm, cand_t, s_max, c_max, k_max) <= budget) { t = cand_t; break; }
+            if (t == 0) t = 8;
+        }
+        const int lds = ceos::coceos_search_lds_bytes(dm, t, s_max, c_max, k_max);
+        if (lds > kMaxLdsBytes)
+            throw ArgError("search parameters need " + std::to_string(lds) +
+                           " bytes of LDS per workgroup (n_cand or n_probed_vectors too large)");
+        ceos::QueryConfig cfg{};
+        cfg.table_log2 = t;
+        cfg.lds_bytes = lds;
+        const int fit = std::max(1, (160 * 1024) / (lds + 2048));
+        cfg.grid = device_cus() * std::min(wg, fit);
+        cfg.s_max = s_max; cfg.c_max = c_max; cfg.k_max = k_max;
+        cfg.cand_stride = (int64_t)s_max * std::max(1, std::min(mp, m)) + c_max;
+        last_lds = lds; last_tlog2 = t; last_wg = std::min(wg, fit);
+        return cfg;
+    }
+
+    ceos::QueryScratch scratch(const ceos::QueryConfig& cfg) {
+        cand.ensure((size_t)cfg.grid * cfg.cand_stride);
+        if (!work.p) work.alloc(1);
+        if (!status.p) { status.alloc(1); HIP_CHECK(hipMemset(status.p, 0, 4)); }
+        return ceos::QueryScratch{cand.p, work.p, status.p};
+    }
+
+    ceos::RowRing rows() const { return ceos::RowRing{X.p, cap, head, base, live}; }
+
+    void check_status() {
+        int32_t flag = 0;
+        HIP_CHECK(hipMemcpy(&flag, status.p, 4, hipMemcpyDeviceToHost));
+        if (flag) {
+            HIP_CHECK(hipMemset(status.p, 0, 4));
+            throw DevError("LDS score tile overflowed; rerun with a larger table (ceos_set_search_config)");
+        }
+    }
+
+    void validate_query(int s_, int mp_, int c_, int k, int s_limit, const char* s_limit_name) const {
+        if (!built) throw StateError("index is not built");
+        if (mp_ > m) throw ArgError("Number of probed points must be smaller than number of indexed top-m points");
+        if (s_ > s_limit)
+            throw ArgError(std::string("Number of probed vectors must be smaller than ") + s_limit_name);
+        if (s_ <= 0 || mp_ <= 0 || c_ <= 0) throw ArgError("n_probed_vectors, n_probed_points and n_cand must be positive");
+        if (k < 0 || k > kMaxK) throw ArgError("n_neighbors must be in [0, " + std::to_string(kMaxK) + "]");
+        if (s_ > kMaxProbes) throw ArgError("n_probed_vectors above " + std::to_string(kMaxProbes) + " is not supported");
+    }
+
+    // Uniform-parameter coCEOs search on device pointers.
+    void search_device(const float* dQ, int nq, int k, int s_, int c_, int32_t* ids, float* ips, hipStream_t s__) {
+        if (nq <= 0 || k == 0) return;
+        const ceos::QueryConfig cfg = search_config(s_, c_, k);
+        const ceos::QueryScratch scr = scratch(cfg);
+        ceos::QueryBatch qb{dQ, nq, nullptr, nullptr, nullptr, k, s_, c_, mp, nullptr, ids, ips};
+        HIP_CHECK(ceos::launch_coceos_search(dims(), signs.p, bkt.p, bsize.p, rows(), qb, cfg, scr, s__));
+    }
+
+    void search_host(const float* Q, int nq, int k, int s_, int c_, int32_t* ids, float* ips) {
+        if (nq <= 0 || k == 0) return;
+        DBuf<float> dQ((size_t)nq * d);
+        DBuf<int32_t> dI((size_t)nq * k);
+        DBuf<float> dP((size_t)nq * k);
+        HIP_CHECK(hipMemcpyAsync(dQ.p, Q, (size_t)nq * d * 4, hipMemcpyHostToDevice, st()));
+        search_device(dQ.p, nq, k, s_, c_, dI.p, dP.p, st());
+        HIP_CHECK(hipMemcpyAsync(ids, dI.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, st()));
+        HIP_CHECK(hipMemcpyAsync(ips, dP.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, st()));
+        HIP_CHECK(hipStreamSynchronize(st()));
+        check_status();
+    }
+
+    void get_buckets(int32_t* cid, float* cval, int32_t* fid, float* fval, int32_t* csz, int32_t* fsz) {
+        if (!built) throw StateError("index is not built");
+        const int DR_ = DR();
+        std::vector<Entry> h((size_t)2 * DR_ * m);
+        std::vector<int32_t> sz(2 * DR_);
+        HIP_CHECK(hipMemcpy(h.data(), bkt.p, h.size() * sizeof(Entry), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(sz.data(), bsize.p, sz.size() * 4, hipMemcpyDeviceToHost));
+        for (int col = 0; col < DR_; ++col) {
+            for (int side = 0; side < 2; ++side) {
+                const int l = 2 * col + side;
+                int32_t* oid = side ? fid : cid;
+                float* ov = side ? fval : cval;
+                for (int t = 0; t < m; ++t) {
+                    const Entry e = h[(size_t)l * m + t];
+                    const bool valid = t < sz[l];
+                    oid[(size_t)col * m + t] = valid ? (int32_t)(e.x - base) : 0;
+                    uint32_t bits_ = valid ? e.y : 0u;
+                    std::memcpy(&ov[(size_t)col * m + t], &bits_, 4);
+                }
+                if (side == 0 && csz) csz[col] = sz[l];
+                if (side == 1 && fsz) fsz[col] = sz[l];
+            }
+        }
+    }
+};
+
+void check_matrix(const void* p, int64_t n, int d, int expect_d) {
+    if (!p && n > 0) throw ArgError("null data pointer");
+    if (n < 0) throw ArgError("negative row count");
+    if (d != expect_d)
+        throw ArgError("feature dimension " + std::to_string(d) + " does not match the index (" +
+                       std::to_string(expect_d) + ")");
+}
+
+}  // namespace
+
+struct ceos_index {
+    Core core;
+    int n_points = 0;
+};
+
+struct ceos_stream {
+    Core core;
+};
+
+extern "C" {
+
+const char* ceos_last_error(void) { return g_err.c_str(); }
+
+int ceos_create(int n_points, int n_features, ceos_index** out) {
+    return guarded([&] {
+        if (!out) throw ArgError("null output handle");
+        if (n_points < 0 || n_features <= 0) throw ArgError("n_points must be >= 0 and n_features > 0");
+        if (n_features > 4096) throw ArgError("n_features above 4096 is not supported");
+        auto* h = new ceos_index();
+        h->n_points = n_points;
+        h->core.d = n_features;
+        *out = h;
+    });
+}
+
+void ceos_destroy(ceos_index* h) { delete h; }
+
+int ceos_set_index_param(ceos_index* h, int n_proj, int n_repeats, int top_m, int n_threads, int seed) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.set_index_param(n_proj, n_repeats, top_m, n_threads, seed);
+        h->core.mp = top_m;  // src/CEOs.h:69
+    });
+}
+
+int ceos_set_query_param(ceos_index* h, int s, int mp, int c) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.s = s; h->core.mp = mp; h->core.c = c;
+    });
+}
+
+int ceos_get_query_param(const ceos_index* h, int* s, int* mp, int* c) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        if (s) *s = h->core.s;
+        if (mp) *mp = h->core.mp;
+        if (c) *c = h->core.c;
+    });
+}
+
+int ceos_set_centering(ceos_index* h, int centering) {
+    return guarded([&] { if (!h) throw ArgError("null handle"); h->core.centering = centering != 0; });
+}
+int ceos_get_centering(const ceos_index* h, int* centering) {
+    return guarded([&] { if (!h || !centering) throw ArgError("null handle"); *centering = h->core.centering; });
+}
+int ceos_set_threads(ceos_index* h, int n_threads) {
+    return guarded([&] { if (!h) throw ArgError("null handle"); h->core.threads = n_threads; });
+}
+int ceos_get_threads(const ceos_index* h, int* n_threads) {
+    return guarded([&] { if (!h || !n_threads) throw ArgError("null handle"); *n_threads = h->core.threads; });
+}
+int ceos_clear(ceos_index* h) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.clear();
+    });
+}
+
+int ceos_get_dims(const ceos_index* h, int* fht, int* n_proj, int* n_repeats, int* top_m, int* seed) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        const Core& c = h->core;
+        if (fht) *fht = c.F;
+        if (n_proj) *n_proj = c.D;
+        if (n_repeats) *n_repeats = c.R;
+        if (top_m) *top_m = c.m;
+        if (seed) *seed = (int)c.seed_used;
+    });
+}
+
+static void build_coceos_impl(ceos_index* h, const float* X, int64_t n, int d, bool dev, hipStream_t st) {
+    Core& c = h->core;
+    if (!c.params_set) throw StateError("setIndexParam must be called before build");
+    check_matrix(X, n, d, c.d);
+    if (n <= 0) throw ArgError("empty dataset");
+    if (n >= (int64_t(1) << 31)) throw ArgError("more than 2^31-1 points");
+    c.built = false;
+    h->n_points = (int)n;
+    c.estP.release();
+    c.load_points(X, n, n, dev, st);
+    c.make_signs();
+    c.build_buckets(n, st);
+    c.built = true;
+}
+
+int ceos_build_coceos(ceos_index* h, const float* X, int64_t n, int d) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        build_coceos_impl(h, X, n, d, false, h->core.st());
+    });
+}
+
+int ceos_build_coceos_device(ceos_index* h, const float* dX, int64_t n, int d, void* stream) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        build_coceos_impl(h, dX, n, d, true, h->core.stream(stream));
+    });
+}
+
+int ceos_search_coceos(ceos_index* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        check_matrix(Q, nq, d, c.d);
+        c.validate_query(c.s, c.mp, c.c, k, c.DR(), "n_proj * n_repeats");
+        c.search_host(Q, nq, k, c.s, c.c, ids, ips);
+    });
+}
+
+int ceos_search_coceos_device(ceos_index* h, const float* dQ, int nq, int d, int k, int32_t* d_ids, float* d_ips,
+                              void* stream) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        check_matrix(dQ, nq, d, c.d);
+        c.validate_query(c.s, c.mp, c.c, k, c.DR(), "n_proj * n_repeats");
+        c.search_device(dQ, nq, k, c.s, c.c, d_ids, d_ips, c.stream(stream));
+    });
+}
+
+int ceos_search_batched_device(ceos_index* h, const float* dQ, int nq, int d, const int32_t* d_k,
+                               const int32_t* d_s, const int32_t* d_c, const int64_t* d_off, int k_max,
+                               int s_max, int c_max, int32_t* d_ids, float* d_ips, void* stream) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        check_matrix(dQ, nq, d, c.d);
+        c.validate_query(s_max, c.mp, c_max, k_max, c.DR(), "n_proj * n_repeats");
+        if (nq <= 0) return;
+        if (!d_k || !d_s || !d_c || !d_off) throw ArgError("per-query arrays must not be null");
+        const ceos::QueryConfig cfg = c.search_config(s_max, c_max, std::max(1, k_max));
+        const ceos::QueryScratch scr = c.scratch(cfg);
+        ceos::QueryBatch qb{dQ, nq, d_k, d_s, d_c, 0, 0, 0, c.mp, d_off, d_ids, d_ips};
+        HIP_CHECK(ceos::launch_coceos_search(c.dims(), c.signs.p, c.bkt.p, c.bsize.p, c.rows(), qb, cfg, scr,
+                                             c.stream(stream)));
+    });
+}
+
+int ceos_search_batched(ceos_index* h, const float* Q, int nq, int d, const int32_t* k, const int32_t* s,
+                        const int32_t* c_, int32_t* ids, float* ips) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        check_matrix(Q, nq, d, c.d);
+        if (nq <= 0) return;
+        if (!k || !s || !c_) throw ArgError("per-query arrays must not be null");
+        int k_max = 0, s_max = 1, c_max = 1;
+        std::vector<int64_t> off(nq);
+        int64_t tot = 0;
+        for (int q = 0; q < nq; ++q) {
+            if (k[q] < 0 || s[q] <= 0 || c_[q] <= 0) throw ArgError("per-query k must be >= 0 and s, c > 0");
+            off[q] = tot;
+            tot += k[q];
+            k_max = std::max(k_max, k[q]);
+            s_max = std::max(s_max, s[q]);
+            c_max = std::max(c_max, c_[q]);
+        }
+        c.validate_query(s_max, c.mp, c_max, k_max, c.DR(), "n_proj * n_repeats");
+        if (tot == 0) return;
+        DBuf<float> dQ((size_t)nq * d);
+        DBuf<int32_t> dk(nq), ds(nq), dc(nq), dI(tot);
+        DBuf<int64_t> doff(nq);
+        DBuf<float> dP(tot);
+        HIP_CHECK(hipMemcpyAsync(dQ.p, Q, (size_t)nq * d * 4, hipMemcpyHostToDevice, c.st()));
+        HIP_CHECK(hipMemcpyAsync(dk.p, k, nq * 4, hipMemcpyHostToDevice, c.st()));
+        HIP_CHECK(hipMemcpyAsync(ds.p, s, nq * 4, hipMemcpyHostToDevice, c.st()));
+        HIP_CHECK(hipMemcpyAsync(dc.p, c_, nq * 4, hipMemcpyHostToDevice, c.st()));
+        HIP_CHECK(hipMemcpyAsync(doff.p, off.data(), nq * 8, hipMemcpyHostToDevice, c.st()));
+        const ceos::QueryConfig cfg = c.search_config(s_max, c_max, std::max(1, k_max));
+        const ceos::QueryScratch scr = c.scratch(cfg);
+        ceos::QueryBatch qb{dQ.p, nq, dk.p, ds.p, dc.p, 0, 0, 0, c.mp, doff.p, dI.p, dP.p};
+        HIP_CHECK(ceos::launch_coceos_search(c.dims(), c.signs.p, c.bkt.p, c.bsize.p, c.rows(), qb, cfg, scr, c.st()));
+        HIP_CHECK(hipMemcpyAsync(ids, dI.p, tot * 4, hipMemcpyDeviceToHost, c.st()));
+        HIP_CHECK(hipMemcpyAsync(ips, dP.p, tot * 4, hipMemcpyDeviceToHost, c.st()));
+        HIP_CHECK(hipStreamSynchronize(c.st()));
+        c.check_status();
+    });
+}
+
+int ceos_build_est(ceos_index* h, const float* X, int64_t n, int d) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        if (!c.params_set) throw StateError("setIndexParam must be called before build");
+        check_matrix(X, n, d, c.d);
+        if (n <= 0) throw ArgError("empty dataset");
+        c.built = false;
+        h->n_points = (int)n;
+        c.bkt.release();
+        c.bsize.release();
+        c.load_points(X, n, n, false, c.st());
+        c.make_signs();
+        c.estP.alloc((size_t)c.DR() * n);
+        c.est_n = n;
+        HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p, 0, 1, n, c.signs.p, c.estP.p, n, true, c.st()));
+        HIP_CHECK(hipStreamSynchronize(c.st()));
+        c.built = true;
+    });
+}
+
+int ceos_search_est(ceos_index* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        check_matrix(Q, nq, d, c.d);
+        if (!c.built || !c.estP.p) throw StateError("CEOs-Est index is not built (call build)");
+        if (c.s > c.DR()) throw ArgError("Number of probed vectors must be smaller than n_proj * n_repeats");
+        if (c.s <= 0 || c.c <= 0) throw ArgError("n_probed_vectors and n_cand must be positive");
+        if (k < 0 || k > kMaxK) throw ArgError("n_neighbors out of range");
+        if (nq <= 0 || k == 0) return;
+        const int cc = (int)std::min<int64_t>(c.c, c.est_n);
+        const int lds = ceos::est_search_lds_bytes(c.dims(), c.s, cc, k);
+        if (lds > kMaxLdsBytes) throw ArgError("n_cand too large for the device path");
+        const int64_t stride = c.est_n + cc;
+        int grid = device_cus() * 2;
+        grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t(2) << 30) / (stride * 8)));
+        c.cand.ensure((size_t)grid * stride);
+        if (!c.work.p) c.work.alloc(1);
+        DBuf<float> dQ((size_t)nq * d);
+        DBuf<int32_t> dI((size_t)nq * k);
+        DBuf<float> dP((size_t)nq * k);
+        HIP_CHECK(hipMemcpyAsync(dQ.p, Q, (size_t)nq * d * 4, hipMemcpyHostToDevice, c.st()));
+        ceos::QueryBatch qb{dQ.p, nq, nullptr, nullptr, nullptr, k, c.s, cc, 0, nullptr, dI.p, dP.p};
+        HIP_CHECK(ceos::launch_est_search(c.dims(), c.signs.p, c.estP.p, c.est_n, c.X.p, qb, grid, c.cand.p, stride,
+                                          c.work.p, c.st()));
+        HIP_CHECK(hipMemcpyAsync(ids, dI.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c.st()));
+        HIP_CHECK(hipMemcpyAsync(ips, dP.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c.st()));
+        HIP_CHECK(hipStreamSynchronize(c.st()));
+    });
+}
+
+int ceos_get_buckets(const ceos_index* h, int32_t* cid, float* cval, int32_t* fid, float* fval) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        auto& c = const_cast<Core&>(h->core);
+        std::lock_guard<std::mutex> g(c.mu);
+        c.get_buckets(cid, cval, fid, fval, nullptr, nullptr);
+    });
+}
+
+int ceos_get_sign_bits(const ceos_index* h, uint8_t* bits) {
+    return guarded([&] {
+        if (!h || !bits) throw ArgError("null handle");
+        if (h->core.bits.empty()) throw StateError("index is not built");
+        std::memcpy(bits, h->core.bits.data(), h->core.bits.size());
+    });
+}
+
+int ceos_project(ceos_index* h, const float* rows, int64_t n, int d, float* out) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        check_matrix(rows, n, d, c.d);
+        if (!c.signs.p) throw StateError("index is not built");
+        if (n == 0) return;
+        DBuf<float> dr((size_t)n * d), dout((size_t)n * c.DR());
+        HIP_CHECK(hipMemcpyAsync(dr.p, rows, (size_t)n * d * 4, hipMemcpyHostToDevice, c.st()));
+        HIP_CHECK(ceos::launch_project_rows(c.dims(), dr.p, 0, 1, n, c.signs.p, dout.p, c.DR(), false, c.st()));
+        HIP_CHECK(hipMemcpyAsync(out, dout.p, (size_t)n * c.DR() * 4, hipMemcpyDeviceToHost, c.st()));
+        HIP_CHECK(hipStreamSynchronize(c.st()));
+    });
+}
+
+int ceos_set_search_config(ceos_index* h, int wg_per_cu, int table_log2) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        if (wg_per_cu < 0 || wg_per_cu > 8) throw ArgError("wg_per_cu must be in [0, 8]");
+        if (table_log2 != 0 && (table_log2 < 8 || table_log2 > 16)) throw ArgError("table_log2 must be 0 or in [8, 16]");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.wg_per_cu = wg_per_cu;
+        h->core.table_log2 = table_log2;
+    });
+}
+
+int ceos_get_search_config(const ceos_index* h, int* wg_per_cu, int* table_log2, int* lds_bytes) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        if (wg_per_cu) *wg_per_cu = h->core.last_wg;
+        if (table_log2) *table_log2 = h->core.last_tlog2;
+        if (lds_bytes) *lds_bytes = h->core.last_lds;
+    });
+}
+
+void* ceos_get_stream(const ceos_index* h) {
+    if (!h) return nullptr;
+    try {
+        return reinterpret_cast<void*>(const_cast<Core&>(h->core).st());
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+// ------------------------------- streamCEOs -------------------------------------
+
+int ceos_stream_create(int n_features, ceos_stream** out) {
+    return guarded([&] {
+        if (!out) throw ArgError("null output handle");
+        if (n_features <= 0 || n_features > 4096) throw ArgError("n_features must be in [1, 4096]");
+        auto* h = new ceos_stream();
+        h->core.d = n_features;
+        *out = h;
+    });
+}
+
+void ceos_stream_destroy(ceos_stream* h) { delete h; }
+
+int ceos_stream_set_index_param(ceos_stream* h, int n_proj, int n_repeats, int top_points, int n_threads,
+                                int seed) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.set_index_param(n_proj, n_repeats, top_points, n_threads, seed);
+        h->core.s = top_points;  // src/streamCEOs.h:58
+    });
+}
+
+int ceos_stream_set_query_param(ceos_stream* h, int s, int mp, int c) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.s = s; h->core.mp = mp; h->core.c = c;
+    });
+}
+
+int ceos_stream_get_query_param(const ceos_stream* h, int* s, int* mp, int* c) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        if (s) *s = h->core.s;
+        if (mp) *mp = h->core.mp;
+        if (c) *c = h->core.c;
+    });
+}
+
+int ceos_stream_set_threads(ceos_stream* h, int n_threads) {
+    return guarded([&] { if (!h) throw ArgError("null handle"); h->core.threads = n_threads; });
+}
+
+int ceos_stream_clear(ceos_stream* h) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.clear();
+    });
+}
+
+int ceos_stream_build(ceos_stream* h, const float* X, int64_t n, int d) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        if (!c.params_set) throw StateError("setIndexParam must be called before build");
+        check_matrix(X, n, d, c.d);
+        if (n <= 0) throw ArgError("empty dataset");
+        if (n >= (int64_t(1) << 31)) throw ArgError("more than 2^31-1 points");
+        c.built = false;
+        const int64_t capacity = std::max<int64_t>(n + n / 4, 1024);
+        c.load_points(X, n, capacity, false, c.st());
+        c.dead.alloc((size_t)(capacity + 31) / 32);
+        HIP_CHECK(hipMemsetAsync(c.dead.p, 0, c.dead.n * 4, c.st()));
+        c.make_signs();
+        c.build_buckets(n, c.st());
+        c.built = true;
+    });
+}
+
+int ceos_stream_update(ceos_stream* h, const float* Xn, int64_t n_new, int d, int n_del) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        if (!c.built) throw StateError("stream index is not built");
+        check_matrix(Xn, n_new, d, c.d);
+        const int64_t n_cur = c.live;
+        if (n_del < 0) throw ArgError("n_delPoints must be >= 0");
+        if (n_del > n_cur) throw ArgError("Number of removed points must be smaller than the current number of points");
+        if (c.m > n_cur + n_new - n_del) throw ArgError("There is not enough indexed top-points for coCEOs after update");
+        if (n_cur + n_new >= (int64_t(1) << 31)) throw ArgError("more than 2^31-1 live points");
+        hipStream_t st = c.st();
+        // grow the ring when the new rows do not fit
+        if (n_cur + n_new > c.cap) {
+            const int64_t ncap = std::max<int64_t>(2 * (n_cur + n_new), 1024);
+            DBuf<float> nx((size_t)ncap * c.d);
+            DBuf<uint32_t> nd((size_t)(ncap + 31) / 32);
+            HIP_CHECK(hipMemsetAsync(nd.p, 0, nd.n * 4, st));
+            const int64_t first = std::min<int64_t>(n_cur, c.cap - c.head);
+            HIP_CHECK(hipMemcpyAsync(nx.p, c.X.p + c.head * c.d, (size_t)first * c.d * 4, hipMemcpyDeviceToDevice, st));
+            if (n_cur > first)
+                HIP_CHECK(hipMemcpyAsync(nx.p + first * c.d, c.X.p, (size_t)(n_cur - first) * c.d * 4,
+                                         hipMemcpyDeviceToDevice, st));
+            // carry random-delete marks over to the new slots
+            if (c.dead.p) {
+                std::vector<uint32_t> od(c.dead.n), ndh(nd.n, 0u);
+                HIP_CHECK(hipMemcpyAsync(od.data(), c.dead.p, od.size() * 4, hipMemcpyDeviceToHost, st));
+                HIP_CHECK(hipStreamSynchronize(st));
+                for (int64_t i = 0; i < n_cur; ++i) {
+                    int64_t sl = c.head + i;
+                    if (sl >= c.cap) sl -= c.cap;
+                    if ((od[sl >> 5] >> (sl & 31)) & 1u) ndh[i >> 5] |= 1u << (i & 31);
+                }
+                HIP_CHECK(hipMemcpyAsync(nd.p, ndh.data(), ndh.size() * 4, hipMemcpyHostToDevice, st));
+            }
+            HIP_CHECK(hipStreamSynchronize(st));
+            c.X = std::move(nx);
+            c.dead = std::move(nd);
+            c.cap = ncap;
+            c.head = 0;
+        }
+        // append rows at slots head+n_cur .. (ring), clearing their delete marks
+        const uint32_t first_abs = c.base + (uint32_t)n_cur;
+        int64_t slot0 = c.head + n_cur;
+        if (slot0 >= c.cap) slot0 -= c.cap;
+        const int64_t seg1 = std::min<int64_t>(n_new, c.cap - slot0);
+        if (n_new > 0) {
+            HIP_CHECK(hipMemcpyAsync(c.X.p + slot0 * c.d, Xn, (size_t)seg1 * c.d * 4, hipMemcpyHostToDevice, st));
+            if (n_new > seg1)
+                HIP_CHECK(hipMemcpyAsync(c.X.p, Xn + seg1 * c.d, (size_t)(n_new - seg1) * c.d * 4,
+                                         hipMemcpyHostToDevice, st));
+            std::vector<uint32_t> dh(c.dead.n);
+            HIP_CHECK(hipMemcpyAsync(dh.data(), c.dead.p, dh.size() * 4, hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipStreamSynchronize(st));
+            for (int64_t j = 0; j < n_new; ++j) {
+                int64_t sl = slot0 + j;
+                if (sl >= c.cap) sl -= c.cap;
+                dh[sl >> 5] &= ~(1u << (sl & 31));
+            }
+            HIP_CHECK(hipMemcpyAsync(c.dead.p, dh.data(), dh.size() * 4, hipMemcpyHostToDevice, st));
+        }
+        // project the new rows: column-major [DR][n_new]
+        DBuf<float> np((size_t)c.DR() * std::max<int64_t>(n_new, 1));
+        if (n_new > 0) {
+            HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p + slot0 * c.d, 0, 1, seg1, c.signs.p, np.p, n_new,
+                                                true, st));
+            if (n_new > seg1)
+                HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p, 0, 1, n_new - seg1, c.signs.p, np.p + seg1,
+                                                    n_new, true, st));
+        }
+        const uint32_t new_base = c.base + (uint32_t)n_del;
+        const uint32_t new_live = (uint32_t)(n_cur + n_new - n_del);
+        HIP_CHECK(ceos::launch_stream_merge(c.DR(), c.m, c.bkt.p, c.bsize.p, np.p, n_new, first_abs, new_base,
+                                            new_live, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        c.base = new_base;
+        c.live = new_live;
+        c.head = (c.head + n_del) % c.cap;
+    });
+}
+
+int ceos_stream_delete_ids(ceos_stream* h, const int32_t* ids, int64_t count) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        if (!c.built) throw StateError("stream index is not built");
+        if (count < 0 || (!ids && count > 0)) throw ArgError("invalid id list");
+        if (count == 0) return;
+        std::vector<uint32_t> dh(c.dead.n);
+        HIP_CHECK(hipMemcpy(dh.data(), c.dead.p, dh.size() * 4, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < count; ++i) {
+            if (ids[i] < 0 || (uint32_t)ids[i] >= c.live) throw ArgError("id " + std::to_string(ids[i]) + " is not live");
+            int64_t sl = c.head + ids[i];
+            if (sl >= c.cap) sl -= c.cap;
+            dh[sl >> 5] |= 1u << (sl & 31);
+        }
+        HIP_CHECK(hipMemcpyAsync(c.dead.p, dh.data(), dh.size() * 4, hipMemcpyHostToDevice, c.st()));
+        HIP_CHECK(ceos::launch_bucket_purge(2 * c.DR(), c.m, c.bkt.p, c.bsize.p, c.dead.p, c.rows(), c.st()));
+        HIP_CHECK(hipStreamSynchronize(c.st()));
+    });
+}
+
+int ceos_stream_estimate_search(ceos_stream* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        check_matrix(Q, nq, d, c.d);
+        c.validate_query(c.s, c.mp, c.c, k, c.D, "number of projections");  // src/streamCEOs.cpp:386
+        c.search_host(Q, nq, k, c.s, c.c, ids, ips);
+    });
+}
+
+int ceos_stream_size(const ceos_stream* h, int64_t* n) {
+    return guarded([&] {
+        if (!h || !n) throw ArgError("null handle");
+        *n = h->core.live;
+    });
+}
+
+int ceos_stream_get_buckets(const ceos_stream* h, int32_t* cid, float* cval, int32_t* fid, float* fval,
+                            int32_t* csz, int32_t* fsz) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        auto& c = const_cast<Core&>(h->core);
+        std::lock_guard<std::mutex> g(c.mu);
+        c.get_buckets(cid, cval, fid, fval, csz, fsz);
+    });
+}
+
+}  // extern "C"

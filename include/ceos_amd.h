@@ -1,0 +1,146 @@
+/*
+ * ceos_amd — C ABI of the MI355X-native coCEOs library (libceos_amd.so).
+ *
+ * This is the drop-in boundary for the reference's Python surface
+ * (python/python_wrapper.cpp:13-75). The pybind11 module `CEOs` shipped beside it
+ * binds these functions with the reference's names; any other FFI (ctypes, cgo,
+ * JNI) can bind them directly. Conventions:
+ *   - every function returns 0 on success or a CEOS_E* code, and then
+ *     ceos_last_error() (per calling thread) describes the failure; the reference
+ *     calls exit(1) in the same situations (src/CEOs.cpp:452-461,
+ *     src/streamCEOs.cpp:179-189,380-390);
+ *   - host-pointer functions take C-contiguous row-major FP32 arrays (rows x d) and
+ *     fill caller-owned outputs; *_device functions take device pointers and a
+ *     hipStream_t passed as void* (NULL = the index's own stream) and do not
+ *     synchronise;
+ *   - output rows are sorted by inner product descending and zero padded
+ *     (id 0, 0.0) when fewer than k candidates exist (src/CEOs.cpp:659-666);
+ *   - the index owns all device memory; builds copy X into HBM
+ *     (src/CEOs.cpp:326).
+ */
+#ifndef CEOS_AMD_H
+#define CEOS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CEOS_OK 0
+#define CEOS_EINVAL 1   /* invalid argument or parameter combination */
+#define CEOS_EDEVICE 2  /* HIP runtime / kernel failure */
+#define CEOS_ESTATE 3   /* index not built / wrong call order */
+#define CEOS_ENOMEM 4   /* device allocation failed or budget exceeded */
+
+typedef struct ceos_index ceos_index;
+typedef struct ceos_stream ceos_stream;
+
+/* Description of the last failure on this thread ("" if none). */
+const char* ceos_last_error(void);
+
+/* ------------------------------ class CEOs ------------------------------ */
+
+/* CEOs(n_points, n_features) — python_wrapper.cpp:15-21, CEOs.h:58-61 */
+int ceos_create(int n_points, int n_features, ceos_index** out);
+void ceos_destroy(ceos_index* h);
+
+/* setIndexParam(n_proj, n_repeats, top_m, n_threads, random_seed) — python_wrapper.cpp:23-30,
+ * CEOs.h:63-81. Also sets n_probed_points = top_m (CEOs.h:69). seed < 0 draws a
+ * time-based seed (Utilities.cpp:23-25). n_threads only drives host-side work. */
+int ceos_set_index_param(ceos_index* h, int n_proj, int n_repeats, int top_m, int n_threads, int seed);
+
+/* Read/write fields n_probed_vectors, n_probed_points, n_cand, centering
+ * (python_wrapper.cpp:32-35) and set_threads / clear (python_wrapper.cpp:36-37). */
+int ceos_set_query_param(ceos_index* h, int n_probed_vectors, int n_probed_points, int n_cand);
+int ceos_get_query_param(const ceos_index* h, int* n_probed_vectors, int* n_probed_points, int* n_cand);
+int ceos_set_centering(ceos_index* h, int centering);
+int ceos_get_centering(const ceos_index* h, int* centering);
+int ceos_set_threads(ceos_index* h, int n_threads);
+int ceos_get_threads(const ceos_index* h, int* n_threads);
+int ceos_clear(ceos_index* h);
+
+/* Index geometry: fhtDim (CEOs.h:76-79), n_proj, n_repeats, top_m and the seed used. */
+int ceos_get_dims(const ceos_index* h, int* fht_dim, int* n_proj, int* n_repeats, int* top_m, int* seed);
+
+/* build_coCEOs_Est(dataset) — python_wrapper.cpp:43, CEOs.cpp:311-441 */
+int ceos_build_coceos(ceos_index* h, const float* X, int64_t n, int d);
+int ceos_build_coceos_device(ceos_index* h, const float* dX, int64_t n, int d, void* stream);
+
+/* search_coCEOs_Est(queries, n_neighbors) — python_wrapper.cpp:44-45, CEOs.cpp:450-694.
+ * Uses the index's n_probed_vectors, n_probed_points and n_cand. ids/ips: nq x k. */
+int ceos_search_coceos(ceos_index* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips);
+int ceos_search_coceos_device(ceos_index* h, const float* dQ, int nq, int d, int k, int32_t* d_ids,
+                              float* d_ips, void* stream);
+
+/* Batched coCEOs search with per-query k[q], n_probed_vectors s[q] and n_cand c[q]
+ * (no reference counterpart: the batched form of search_coCEOs_Est). Query q's
+ * results are written at off[q] .. off[q]+k[q]-1 of ids/ips, where
+ * off[q] = k[0] + ... + k[q-1]. k_max/s_max/c_max bound the arrays' values. */
+int ceos_search_batched(ceos_index* h, const float* Q, int nq, int d, const int32_t* k, const int32_t* s,
+                        const int32_t* c, int32_t* ids, float* ips);
+int ceos_search_batched_device(ceos_index* h, const float* dQ, int nq, int d, const int32_t* d_k,
+                               const int32_t* d_s, const int32_t* d_c, const int64_t* d_off, int k_max,
+                               int s_max, int c_max, int32_t* d_ids, float* d_ips, void* stream);
+
+/* build(dataset) / search(queries, n_neighbors) = CEOs-Est — python_wrapper.cpp:39-41,
+ * CEOs.cpp:18-98 and 107-297 (dense n x n_proj*n_repeats projections). */
+int ceos_build_est(ceos_index* h, const float* X, int64_t n, int d);
+int ceos_search_est(ceos_index* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips);
+
+/* Introspection used by parity tests. Buckets are [n_proj*n_repeats][top_m] each:
+ * close ids/values then far ids/values (far values hold -projection), the layout of
+ * matrix_P rows (CEOs.cpp:417-428). Sign bits: fhtDim*3*n_repeats bytes (0/1) in the
+ * order of bitHD3Generator (Utilities.cpp:21-41). Projections: n x (n_proj*n_repeats). */
+int ceos_get_buckets(const ceos_index* h, int32_t* close_ids, float* close_vals, int32_t* far_ids,
+                     float* far_vals);
+int ceos_get_sign_bits(const ceos_index* h, uint8_t* bits);
+int ceos_project(ceos_index* h, const float* rows, int64_t n, int d, float* out);
+
+/* Tuning of the search launch (no reference counterpart): workgroups per CU and log2
+ * of the LDS score-tile slots; 0 = automatic. */
+int ceos_set_search_config(ceos_index* h, int wg_per_cu, int table_log2);
+int ceos_get_search_config(const ceos_index* h, int* wg_per_cu, int* table_log2, int* lds_bytes);
+
+/* hipStream_t the index launches on (as void*). */
+void* ceos_get_stream(const ceos_index* h);
+
+/* --------------------------- class streamCEOs --------------------------- */
+
+/* streamCEOs(n_features) — python_wrapper.cpp:53, streamCEOs.h:47-51 */
+int ceos_stream_create(int n_features, ceos_stream** out);
+void ceos_stream_destroy(ceos_stream* h);
+
+/* setIndexParam(n_proj, n_repeats, top_points, n_threads, random_seed) —
+ * python_wrapper.cpp:55-57, streamCEOs.h:53-70 (sets n_probed_vectors = top_points). */
+int ceos_stream_set_index_param(ceos_stream* h, int n_proj, int n_repeats, int top_points, int n_threads,
+                                int seed);
+int ceos_stream_set_query_param(ceos_stream* h, int n_probed_vectors, int n_probed_points, int n_cand);
+int ceos_stream_get_query_param(const ceos_stream* h, int* n_probed_vectors, int* n_probed_points,
+                                int* n_cand);
+int ceos_stream_set_threads(ceos_stream* h, int n_threads);
+int ceos_stream_clear(ceos_stream* h);
+
+/* build(dataset) — python_wrapper.cpp:65, streamCEOs.cpp:9-161 */
+int ceos_stream_build(ceos_stream* h, const float* X, int64_t n, int d);
+/* update(new_dataset, n_delPoints) — python_wrapper.cpp:66, streamCEOs.cpp:174-369:
+ * appends the rows, drops the n_del oldest (ids shift down by n_del). */
+int ceos_stream_update(ceos_stream* h, const float* Xnew, int64_t n_new, int d, int n_del);
+/* estimate_search(queries, n_neighbors) — python_wrapper.cpp:67-68, streamCEOs.cpp:378-612 */
+int ceos_stream_estimate_search(ceos_stream* h, const float* Q, int nq, int d, int k, int32_t* ids,
+                                float* ips);
+/* Random delete (no reference counterpart; the reference deletes FIFO only): the
+ * points with these current ids leave every bucket at once and are never returned
+ * again; ids of other points do not change. The rows keep their positions until the
+ * FIFO delete of update() drops them. */
+int ceos_stream_delete_ids(ceos_stream* h, const int32_t* ids, int64_t count);
+int ceos_stream_size(const ceos_stream* h, int64_t* n_points);
+/* Buckets with per-list valid sizes ([n_proj*n_repeats] each). */
+int ceos_stream_get_buckets(const ceos_stream* h, int32_t* close_ids, float* close_vals, int32_t* far_ids,
+                            float* far_vals, int32_t* close_size, int32_t* far_size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CEOS_AMD_H */

@@ -1,0 +1,510 @@
+// CPU oracle for coCEOs / CEOs-Est / streamCEOs.
+//
+// TEST INFRASTRUCTURE ONLY. This file is a plain C++17/OpenMP restatement of the
+// reference algorithms, written from reading them. Only the test suite, the smoke
+// check and the CPU-baseline leg of bench.py load it, always as the checker; the
+// product (ceos_amd/) never links or calls it.
+//
+// Every function names the reference file:line it restates (paths are relative to
+// the reference repository root). Tie behaviour: the reference keeps bounded
+// min-heaps `priority_queue<IFPair, vector<IFPair>, greater<>>` that compare the
+// value only (src/Header.h:53-80); this oracle uses the same std::priority_queue
+// with the same value-only comparator, so bucket and top-k tie outcomes follow the
+// same libstdc++ heap algorithms. Two documented differences, both visible only
+// at bit-identical scores:
+//   * candidate selection feeds the heap in first-insertion order of the score map,
+//     where the reference iterates a tsl::robin_map (src/CEOs.cpp:619);
+//   * the rerank dot product is evaluated in FP64 and rounded once to FP32, where
+//     the reference uses Eigen's SIMD dot (src/CEOs.cpp:646), whose summation order
+//     depends on the Eigen version and ISA.
+// Where the reference is undefined (popping an empty heap when fewer than n_cand
+// distinct candidates exist, src/CEOs.cpp:640-645; stale bucket slots after a
+// stream update, src/streamCEOs.cpp:337-351) the oracle uses the defined rule the
+// product uses: take min(n_cand, distinct) candidates; a bucket holds only its
+// valid entries.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <queue>
+#include <random>
+#include <unordered_map>
+#include <vector>
+
+#include <omp.h>
+
+namespace {
+
+struct IFPair {  // (id, value) ordered by value only: src/Header.h:53-80
+    int id;
+    float v;
+    IFPair(int i, float x) : id(i), v(x) {}
+    bool operator<(const IFPair& o) const { return v < o.v; }
+    bool operator>(const IFPair& o) const { return v > o.v; }
+};
+using MinHeap = std::priority_queue<IFPair, std::vector<IFPair>, std::greater<>>;
+
+// src/Utilities.cpp:21-41 (bitHD3Generator): std::default_random_engine seeded with
+// `seed` and uniform_int_distribution<uint32_t>(0,1), one draw per bit.
+void sign_bits(int nbits, int seed, uint8_t* out) {
+    unsigned s = (unsigned)seed;
+    std::default_random_engine gen(s);
+    std::uniform_int_distribution<uint32_t> dist(0, 1);
+    for (int i = 0; i < nbits; ++i) out[i] = (uint8_t)(dist(gen) & 1u);
+}
+
+// In-place unnormalised Walsh-Hadamard transform with butterfly stages applied in
+// ascending bit order (stride 1, 2, 4, ...), each butterfly (u+v, u-v). This is the
+// stage order of the reference's fht_float (src/fht_avx.c:1012-1252 for 2^10).
+template <typename T>
+void fwht(T* buf, int log_n) {
+    const int n = 1 << log_n;
+    for (int h = 1; h < n; h <<= 1)
+        for (int i = 0; i < n; i += 2 * h)
+            for (int j = i; j < i + h; ++j) {
+                T u = buf[j], v = buf[j + h];
+                buf[j] = u + v;
+                buf[j + h] = u - v;
+            }
+}
+
+// src/CEOs.h:76-79
+int fht_dim(int n_proj, int d) {
+    if (n_proj < d) return 1 << (int)std::ceil(std::log2((double)d));
+    return 1 << (int)std::ceil(std::log2((double)n_proj));
+}
+
+struct Params {
+    int n = 0, d = 0, D = 256, R = 1, m = 100, threads = 1, seed = -1, F = 0;
+    std::vector<uint8_t> bits;  // F * 3 * R
+};
+
+// src/CEOs.cpp:343-372: zero-pad to F, then for each repeat 3 x (sign, FWHT); keep the
+// first D coordinates of each repeat.
+void project_row(const Params& P, const float* x, float* out /* D*R */) {
+    const int L = (int)std::log2((double)P.F);
+    std::vector<float> base(P.F, 0.0f), y(P.F);
+    std::memcpy(base.data(), x, sizeof(float) * P.d);
+    for (int r = 0; r < P.R; ++r) {
+        y = base;
+        const size_t b0 = (size_t)P.F * 3 * r;
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < P.F; ++j)
+                y[j] *= (2 * (float)P.bits[b0 + (size_t)i * P.F + j] - 1);
+            fwht(y.data(), L);
+        }
+        std::memcpy(out + (size_t)P.D * r, y.data(), sizeof(float) * P.D);
+    }
+}
+
+void project_all(const Params& P, const float* X, int64_t n, std::vector<float>& proj) {
+    const int DR = P.D * P.R;
+    proj.assign((size_t)n * DR, 0.0f);
+#pragma omp parallel for num_threads(P.threads) schedule(static)
+    for (int64_t i = 0; i < n; ++i) project_row(P, X + (size_t)i * P.d, proj.data() + (size_t)i * DR);
+}
+
+// src/CEOs.cpp:377-432: per column, two bounded min-heaps (close: value, far: -value),
+// replacement only on strict '>', drained largest-first.
+struct Buckets {
+    int DR = 0, m = 0;
+    std::vector<int32_t> cid, fid;   // [DR][m]
+    std::vector<float> cval, fval;   // [DR][m]; far values hold -projection
+    std::vector<int32_t> csize, fsize;  // valid entries per list (== m after a build)
+    void alloc(int dr, int mm) {
+        DR = dr; m = mm;
+        cid.assign((size_t)DR * m, 0); fid.assign((size_t)DR * m, 0);
+        cval.assign((size_t)DR * m, 0.f); fval.assign((size_t)DR * m, 0.f);
+        csize.assign(DR, 0); fsize.assign(DR, 0);
+    }
+};
+
+void drain(MinHeap& h, int32_t* ids, float* vals, int32_t id_shift) {
+    for (int t = (int)h.size() - 1; t >= 0; --t) {
+        ids[t] = h.top().id - id_shift;
+        vals[t] = h.top().v;
+        h.pop();
+    }
+}
+
+void build_buckets(const Params& P, const std::vector<float>& proj, int64_t n, Buckets& B) {
+    const int DR = P.D * P.R;
+    B.alloc(DR, P.m);
+#pragma omp parallel for num_threads(P.threads) schedule(dynamic, 1)
+    for (int c = 0; c < DR; ++c) {
+        MinHeap close, far;
+        for (int64_t i = 0; i < n; ++i) {
+            const float v = proj[(size_t)i * DR + c];
+            if ((int)close.size() < P.m) close.emplace((int)i, v);
+            else if (v > close.top().v) { close.pop(); close.emplace((int)i, v); }
+            if ((int)far.size() < P.m) far.emplace((int)i, -v);
+            else if (-v > far.top().v) { far.pop(); far.emplace((int)i, -v); }
+        }
+        B.csize[c] = (int)close.size();
+        B.fsize[c] = (int)far.size();
+        drain(close, &B.cid[(size_t)c * P.m], &B.cval[(size_t)c * P.m], 0);
+        drain(far, &B.fid[(size_t)c * P.m], &B.fval[(size_t)c * P.m], 0);
+    }
+}
+
+// src/CEOs.cpp:517-548: top-s columns by |p|, encoded +(c+1) (close) / -(c+1) (far,
+// when p < 0); popped afterwards in ascending |p| order (src/CEOs.cpp:562-564).
+std::vector<int> probe_order(const float* p, int DR, int s) {
+    MinHeap h;
+    for (int c = 0; c < DR; ++c) {
+        float a = p[c];
+        int code = c + 1;
+        if (a < 0) { a = -a; code = -code; }
+        if ((int)h.size() < s) h.emplace(code, a);
+        else if (a > h.top().v) { h.pop(); h.emplace(code, a); }
+    }
+    std::vector<int> order;
+    while (!h.empty()) { order.push_back(h.top().id); h.pop(); }
+    return order;
+}
+
+double dot64(const float* a, const float* b, int d) {
+    double acc = 0.0;
+    for (int j = 0; j < d; ++j) acc += (double)a[j] * (double)b[j];
+    return acc;
+}
+
+// src/CEOs.cpp:638-666: rerank candidates popped in ascending estimate order, keep
+// top-k by strict '>', write the row descending, zero-padded.
+void rerank(const float* q, int d, const std::vector<int>& cands_asc,
+            const std::function<const float*(int)>& row, int k, int32_t* out_id, float* out_ip) {
+    MinHeap top;
+    for (int id : cands_asc) {
+        const float ip = (float)dot64(q, row(id), d);
+        if ((int)top.size() < k) top.emplace(id, ip);
+        else if (ip > top.top().v) { top.pop(); top.emplace(id, ip); }
+    }
+    for (int t = 0; t < k; ++t) { out_id[t] = 0; out_ip[t] = 0.0f; }
+    for (int t = (int)top.size() - 1; t >= 0; --t) {
+        out_id[t] = top.top().id; out_ip[t] = top.top().v; top.pop();
+    }
+}
+
+// Score accumulation (src/CEOs.cpp:558-610) in probe pop order, then the bounded
+// candidate heap (src/CEOs.cpp:619-632) fed in first-insertion order.
+struct QueryStats { int distinct; float margin; };
+
+void search_one(const Params& P, const Buckets& B, const float* q, int k, int s, int mp, int c,
+                const std::function<const float*(int)>& row, int32_t* out_id, float* out_ip,
+                QueryStats* st) {
+    const int DR = P.D * P.R;
+    std::vector<float> proj(DR);
+    project_row(P, q, proj.data());
+    std::vector<int> probes = probe_order(proj.data(), DR, s);
+
+    std::unordered_map<int, int> slot;
+    slot.reserve((size_t)s * mp * 2);
+    std::vector<int> ids;
+    std::vector<float> sums;
+    for (int code : probes) {
+        const bool far = code < 0;
+        const int col = (far ? -code : code) - 1;
+        const int32_t* lid = far ? &B.fid[(size_t)col * B.m] : &B.cid[(size_t)col * B.m];
+        const float* lv = far ? &B.fval[(size_t)col * B.m] : &B.cval[(size_t)col * B.m];
+        const int valid = far ? B.fsize[col] : B.csize[col];
+        const int lim = std::min(mp, valid);
+        for (int t = 0; t < lim; ++t) {
+            auto it = slot.find(lid[t]);
+            if (it == slot.end()) {
+                slot.emplace(lid[t], (int)ids.size());
+                ids.push_back(lid[t]);
+                sums.push_back(lv[t]);
+            } else {
+                sums[it->second] += lv[t];
+            }
+        }
+    }
+    MinHeap cand;
+    for (size_t i = 0; i < ids.size(); ++i) {
+        if ((int)cand.size() < c) cand.emplace(ids[i], sums[i]);
+        else if (sums[i] > cand.top().v) { cand.pop(); cand.emplace(ids[i], sums[i]); }
+    }
+    if (st) {
+        st->distinct = (int)ids.size();
+        // gap between the weakest kept estimate and the best rejected one
+        float best_rej = -INFINITY;
+        if ((int)ids.size() > c) {
+            std::vector<float> all(sums);
+            std::nth_element(all.begin(), all.begin() + c, all.end(), std::greater<float>());
+            best_rej = all[c];
+        }
+        st->margin = cand.empty() ? INFINITY : cand.top().v - best_rej;
+    }
+    std::vector<int> asc;
+    while (!cand.empty()) { asc.push_back(cand.top().id); cand.pop(); }
+    rerank(q, P.d, asc, row, k, out_id, out_ip);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------
+// C ABI (ctypes) — oracle handles mirror the reference classes CEOs and streamCEOs.
+// ------------------------------------------------------------------------------
+
+static void make_bits(Params& P) {
+    P.bits.assign((size_t)P.F * 3 * P.R, 0);
+    sign_bits(P.F * 3 * P.R, P.seed, P.bits.data());
+}
+
+struct OracleCEOs {
+    Params P;
+    std::vector<float> X;      // n x d (src/CEOs.cpp:326 copies the input)
+    Buckets B;                 // coCEOs buckets (src/CEOs.cpp:332)
+    std::vector<float> est;    // CEOs-Est projections, n x DR row-major (src/CEOs.cpp:52)
+};
+
+struct OracleStream {
+    Params P;
+    std::deque<std::vector<float>> X;  // src/streamCEOs.h:26
+    Buckets B;
+};
+
+extern "C" {
+
+void oracle_sign_bits(int nbits, int seed, uint8_t* out) { sign_bits(nbits, seed, out); }
+void oracle_fwht_f32(float* buf, int log_n) { fwht(buf, log_n); }
+void oracle_fwht_f64(double* buf, int log_n) { fwht(buf, log_n); }
+int oracle_fht_dim(int n_proj, int d) { return fht_dim(n_proj, d); }
+
+void* oracle_ceos_new(int n, int d) {
+    auto* h = new OracleCEOs();
+    h->P.n = n; h->P.d = d;
+    return h;
+}
+void oracle_ceos_free(void* h) { delete (OracleCEOs*)h; }
+
+// src/CEOs.h:63-81 (set_CEOsParam)
+void oracle_ceos_set_index_param(void* hv, int D, int R, int m, int threads, int seed) {
+    auto* h = (OracleCEOs*)hv;
+    h->P.D = D; h->P.R = R; h->P.m = m; h->P.seed = seed;
+    h->P.threads = threads > 0 ? threads : omp_get_max_threads();
+    h->P.F = fht_dim(D, h->P.d);
+    if (seed >= 0) make_bits(h->P);  // deterministic: the same bits build() draws
+}
+int oracle_ceos_fht_dim(void* hv) { return ((OracleCEOs*)hv)->P.F; }
+
+
+// src/CEOs.cpp:311-441 (build_coCEOs_Est)
+void oracle_ceos_build_coceos(void* hv, const float* X) {
+    auto* h = (OracleCEOs*)hv;
+    Params& P = h->P;
+    h->X.assign(X, X + (size_t)P.n * P.d);
+    make_bits(P);
+    std::vector<float> proj;
+    project_all(P, h->X.data(), P.n, proj);
+    build_buckets(P, proj, P.n, h->B);
+}
+
+void oracle_ceos_get_buckets(void* hv, int32_t* cid, float* cval, int32_t* fid, float* fval) {
+    auto* h = (OracleCEOs*)hv;
+    const size_t sz = h->B.cid.size();
+    std::memcpy(cid, h->B.cid.data(), sz * 4); std::memcpy(cval, h->B.cval.data(), sz * 4);
+    std::memcpy(fid, h->B.fid.data(), sz * 4); std::memcpy(fval, h->B.fval.data(), sz * 4);
+}
+
+void oracle_ceos_get_bits(void* hv, uint8_t* out) {
+    auto* h = (OracleCEOs*)hv;
+    std::memcpy(out, h->P.bits.data(), h->P.bits.size());
+}
+
+// Projections of arbitrary rows with the index's sign diagonals (n x DR row-major).
+void oracle_ceos_project(void* hv, const float* rows, int64_t n, float* out) {
+    auto* h = (OracleCEOs*)hv;
+    const int DR = h->P.D * h->P.R;
+#pragma omp parallel for num_threads(h->P.threads) schedule(static)
+    for (int64_t i = 0; i < n; ++i) project_row(h->P, rows + (size_t)i * h->P.d, out + (size_t)i * DR);
+}
+
+// src/CEOs.cpp:450-694 (search_coCEOs_Est). k, s, c may differ per query (arrays of
+// length nq); outputs are packed with per-query offsets out_off[q] (k values each).
+// Returns 0, or 1 when a parameter check of src/CEOs.cpp:452-461 fails.
+int oracle_ceos_search_coceos(void* hv, const float* Q, int nq, const int* k, const int* s, int mp,
+                              const int* c, const int64_t* out_off, int32_t* out_id, float* out_ip,
+                              int* distinct, float* margin) {
+    auto* h = (OracleCEOs*)hv;
+    const Params& P = h->P;
+    if (mp > P.m) return 1;
+    for (int q = 0; q < nq; ++q) if (s[q] > P.D * P.R) return 1;
+    auto row = [h](int id) { return h->X.data() + (size_t)id * h->P.d; };
+#pragma omp parallel for num_threads(P.threads) schedule(dynamic, 4)
+    for (int q = 0; q < nq; ++q) {
+        QueryStats st{};
+        search_one(P, h->B, Q + (size_t)q * P.d, k[q], s[q], mp, c[q], row, out_id + out_off[q],
+                   out_ip + out_off[q], &st);
+        if (distinct) distinct[q] = st.distinct;
+        if (margin) margin[q] = st.margin;
+    }
+    return 0;
+}
+
+// src/CEOs.cpp:18-98 (build_CEOs): keeps all n x DR projections.
+void oracle_ceos_build_est(void* hv, const float* X) {
+    auto* h = (OracleCEOs*)hv;
+    Params& P = h->P;
+    h->X.assign(X, X + (size_t)P.n * P.d);
+    make_bits(P);
+    project_all(P, h->X.data(), P.n, h->est);
+}
+
+// src/CEOs.cpp:107-297 (search_CEOs): dense estimate over all n, summed over probes in
+// pop order; candidate heap over n in index order; rerank as coCEOs.
+void oracle_ceos_search_est(void* hv, const float* Q, int nq, int k, int s, int c, int32_t* out_id,
+                            float* out_ip) {
+    auto* h = (OracleCEOs*)hv;
+    const Params& P = h->P;
+    const int DR = P.D * P.R;
+    auto row = [h](int id) { return h->X.data() + (size_t)id * h->P.d; };
+#pragma omp parallel for num_threads(P.threads) schedule(dynamic, 4)
+    for (int q = 0; q < nq; ++q) {
+        std::vector<float> proj(DR);
+        project_row(P, Q + (size_t)q * P.d, proj.data());
+        std::vector<int> probes = probe_order(proj.data(), DR, s);
+        std::vector<float> est(P.n, 0.0f);
+        for (int code : probes) {
+            const bool far = code < 0;
+            const int col = (far ? -code : code) - 1;
+            for (int i = 0; i < P.n; ++i) {
+                const float v = h->est[(size_t)i * DR + col];
+                if (far) est[i] -= v; else est[i] += v;
+            }
+        }
+        MinHeap cand;
+        for (int i = 0; i < P.n; ++i) {
+            if ((int)cand.size() < c) cand.emplace(i, est[i]);
+            else if (est[i] > cand.top().v) { cand.pop(); cand.emplace(i, est[i]); }
+        }
+        std::vector<int> asc;
+        while (!cand.empty()) { asc.push_back(cand.top().id); cand.pop(); }
+        rerank(Q + (size_t)q * P.d, P.d, asc, row, k, out_id + (size_t)q * k, out_ip + (size_t)q * k);
+    }
+}
+
+// Brute-force exact MIPS (FP64 dots): the recall ground truth (test/utils.py:35-56
+// uses faiss IndexFlatIP for the same purpose).
+void oracle_exact_mips(const float* X, int64_t n, int d, const float* Q, int nq, int k, int threads,
+                       int32_t* out_id, float* out_ip) {
+#pragma omp parallel for num_threads(threads > 0 ? threads : omp_get_max_threads()) schedule(dynamic, 1)
+    for (int q = 0; q < nq; ++q) {
+        std::priority_queue<std::pair<double, int64_t>, std::vector<std::pair<double, int64_t>>,
+                            std::greater<>> top;
+        for (int64_t i = 0; i < n; ++i) {
+            const double ip = dot64(Q + (size_t)q * d, X + (size_t)i * d, d);
+            if ((int)top.size() < k) top.emplace(ip, -i);
+            else if (std::make_pair(ip, -i) > top.top()) { top.pop(); top.emplace(ip, -i); }
+        }
+        for (int t = (int)top.size() - 1; t >= 0; --t) {
+            out_id[(size_t)q * k + t] = (int32_t)(-top.top().second);
+            out_ip[(size_t)q * k + t] = (float)top.top().first;
+            top.pop();
+        }
+    }
+}
+
+// ---------------------------- streamCEOs ----------------------------------------
+
+void* oracle_stream_new(int d) {
+    auto* h = new OracleStream();
+    h->P.d = d;
+    return h;
+}
+void oracle_stream_free(void* h) { delete (OracleStream*)h; }
+
+// src/streamCEOs.h:53-70 (also sets n_probed_vectors := top_m on the Python object).
+void oracle_stream_set_index_param(void* hv, int D, int R, int m, int threads, int seed) {
+    auto* h = (OracleStream*)hv;
+    h->P.D = D; h->P.R = R; h->P.m = m; h->P.seed = seed;
+    h->P.threads = threads > 0 ? threads : omp_get_max_threads();
+    h->P.F = fht_dim(D, h->P.d);
+}
+
+// src/streamCEOs.cpp:9-161 (build, centering off)
+void oracle_stream_build(void* hv, const float* X, int64_t n) {
+    auto* h = (OracleStream*)hv;
+    Params& P = h->P;
+    h->X.clear();
+    for (int64_t i = 0; i < n; ++i) h->X.emplace_back(X + (size_t)i * P.d, X + (size_t)(i + 1) * P.d);
+    P.n = (int)n;
+    make_bits(P);
+    std::vector<float> proj;
+    project_all(P, X, n, proj);
+    build_buckets(P, proj, n, h->B);
+}
+
+// src/streamCEOs.cpp:174-369 (update): append new rows, drop the n_del oldest (FIFO),
+// per column keep the top-m of {surviving old bucket entries} U {new points}, then
+// renumber ids by -n_del. Returns 1 on the reference's exit(1) conditions.
+int oracle_stream_update(void* hv, const float* Xn, int64_t n_new, int n_del) {
+    auto* h = (OracleStream*)hv;
+    Params& P = h->P;
+    const int n_cur = (int)h->X.size();
+    if (n_del > n_cur) return 1;
+    if (P.m > n_cur + n_new - n_del) return 1;
+    for (int64_t i = 0; i < n_new; ++i) h->X.emplace_back(Xn + (size_t)i * P.d, Xn + (size_t)(i + 1) * P.d);
+    for (int i = 0; i < n_del; ++i) h->X.pop_front();
+    P.n = (int)h->X.size();
+    std::vector<float> proj;
+    project_all(P, Xn, n_new, proj);
+    const int DR = P.D * P.R;
+    Buckets& B = h->B;
+#pragma omp parallel for num_threads(P.threads) schedule(dynamic, 1)
+    for (int c = 0; c < DR; ++c) {
+        MinHeap close, far;
+        for (int t = 0; t < B.csize[c]; ++t) {
+            const int id = B.cid[(size_t)c * B.m + t];
+            if (id >= n_del) close.emplace(id, B.cval[(size_t)c * B.m + t]);
+        }
+        for (int t = 0; t < B.fsize[c]; ++t) {
+            const int id = B.fid[(size_t)c * B.m + t];
+            if (id >= n_del) far.emplace(id, B.fval[(size_t)c * B.m + t]);
+        }
+        for (int64_t i = 0; i < n_new; ++i) {
+            const float v = proj[(size_t)i * DR + c];
+            const int id = (int)(i + n_cur);
+            if ((int)close.size() < P.m) close.emplace(id, v);
+            else if (v > close.top().v) { close.pop(); close.emplace(id, v); }
+            if ((int)far.size() < P.m) far.emplace(id, -v);
+            else if (-v > far.top().v) { far.pop(); far.emplace(id, -v); }
+        }
+        B.csize[c] = (int)close.size();
+        B.fsize[c] = (int)far.size();
+        drain(close, &B.cid[(size_t)c * B.m], &B.cval[(size_t)c * B.m], n_del);
+        drain(far, &B.fid[(size_t)c * B.m], &B.fval[(size_t)c * B.m], n_del);
+    }
+    return 0;
+}
+
+void oracle_stream_get_buckets(void* hv, int32_t* cid, float* cval, int32_t* fid, float* fval,
+                               int32_t* csize, int32_t* fsize) {
+    auto* h = (OracleStream*)hv;
+    const size_t sz = h->B.cid.size();
+    std::memcpy(cid, h->B.cid.data(), sz * 4); std::memcpy(cval, h->B.cval.data(), sz * 4);
+    std::memcpy(fid, h->B.fid.data(), sz * 4); std::memcpy(fval, h->B.fval.data(), sz * 4);
+    std::memcpy(csize, h->B.csize.data(), h->B.DR * 4);
+    std::memcpy(fsize, h->B.fsize.data(), h->B.DR * 4);
+}
+
+int oracle_stream_size(void* hv) { return (int)((OracleStream*)hv)->X.size(); }
+
+// src/streamCEOs.cpp:378-612 (estimate_search); guard s <= n_proj (src/streamCEOs.cpp:386).
+int oracle_stream_search(void* hv, const float* Q, int nq, int k, int s, int mp, int c,
+                         int32_t* out_id, float* out_ip) {
+    auto* h = (OracleStream*)hv;
+    const Params& P = h->P;
+    if (mp > P.m || s > P.D) return 1;
+    auto row = [h](int id) { return h->X[id].data(); };
+#pragma omp parallel for num_threads(P.threads) schedule(dynamic, 4)
+    for (int q = 0; q < nq; ++q)
+        search_one(P, h->B, Q + (size_t)q * P.d, k, s, mp, c, row, out_id + (size_t)q * k,
+                   out_ip + (size_t)q * k, nullptr);
+    return 0;
+}
+
+}  // extern "C"
