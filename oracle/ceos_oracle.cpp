@@ -303,6 +303,23 @@ void oracle_ceos_build_coceos(void* hv, const float* X) {
     build_buckets(P, proj, P.n, h->B);
 }
 
+// Installs points and buckets produced elsewhere (e.g. a device build whose buckets
+// were verified equal to this oracle's), so the CPU search can be timed on them.
+void oracle_ceos_load_index(void* hv, const float* X, const int32_t* cid, const float* cval,
+                            const int32_t* fid, const float* fval) {
+    auto* h = (OracleCEOs*)hv;
+    Params& P = h->P;
+    h->X.assign(X, X + (size_t)P.n * P.d);
+    if (P.bits.empty()) make_bits(P);
+    const int DR = P.D * P.R;
+    h->B.alloc(DR, P.m);
+    const size_t sz = (size_t)DR * P.m;
+    std::memcpy(h->B.cid.data(), cid, sz * 4); std::memcpy(h->B.cval.data(), cval, sz * 4);
+    std::memcpy(h->B.fid.data(), fid, sz * 4); std::memcpy(h->B.fval.data(), fval, sz * 4);
+    std::fill(h->B.csize.begin(), h->B.csize.end(), P.m);
+    std::fill(h->B.fsize.begin(), h->B.fsize.end(), P.m);
+}
+
 void oracle_ceos_get_buckets(void* hv, int32_t* cid, float* cval, int32_t* fid, float* fval) {
     auto* h = (OracleCEOs*)hv;
     const size_t sz = h->B.cid.size();

@@ -51,6 +51,7 @@ def lib():
             "oracle_ceos_build_coceos": (None, [vp, _f32p]),
             "oracle_ceos_get_buckets": (None, [vp, _i32p, _f32p, _i32p, _f32p]),
             "oracle_ceos_get_bits": (None, [vp, _u8p]),
+            "oracle_ceos_load_index": (None, [vp, _f32p, _i32p, _f32p, _i32p, _f32p]),
             "oracle_ceos_project": (None, [vp, _f32p, i64, _f32p]),
             "oracle_ceos_search_coceos": (i32, [vp, _f32p, i32, _i32p, _i32p, i32, _i32p, _i64p,
                                                  _i32p, _f32p, _i32p, _f32p]),
@@ -150,6 +151,12 @@ class OracleCEOs:
         cv = np.zeros((DR, m), np.float32); fv = np.zeros((DR, m), np.float32)
         lib().oracle_ceos_get_buckets(self._h, cid, cv, fid, fv)
         return cid, cv, fid, fv
+
+    def load_index(self, X, cid, cval, fid, fval):
+        """Installs points and buckets built elsewhere (same layout as buckets())."""
+        c = lambda a, t: np.ascontiguousarray(a, dtype=t)
+        lib().oracle_ceos_load_index(self._h, _f32(X), c(cid, np.int32), c(cval, np.float32),
+                                     c(fid, np.int32), c(fval, np.float32))
 
     def sign_bits(self):
         out = np.zeros(self.F * 3 * self.R, np.uint8)

@@ -1,0 +1,69 @@
+"""CPU checks of the C ABI and the drop-in module: the library loads, exports every
+symbol include/ceos_amd.h declares, and the host-side logic (defaults, parameter
+rules, error mapping) mirrors the reference without touching a GPU."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from ceos_amd import capi
+
+
+def test_header_symbols_exported():
+    names = capi.header_symbols()
+    assert len(names) >= 35
+    L = C.CDLL(capi.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_error_codes_without_device():
+    L = capi.lib()
+    h = C.c_void_p()
+    assert L.ceos_create(100, 16, C.byref(h)) == 0
+    try:
+        assert L.ceos_set_index_param(h, 0, 1, 10, -1, 1) == 1          # n_proj must be positive
+        assert L.ceos_set_index_param(h, 8192, 1, 10, -1, 1) == 1       # fhtDim above 4096
+        assert b"4096" in L.ceos_last_error()
+        assert L.ceos_set_index_param(h, 64, 2, 10, -1, 42) == 0
+        q = np.zeros((2, 16), np.float32)
+        ids = np.zeros((2, 5), np.int32)
+        ips = np.zeros((2, 5), np.float32)
+        rc = L.ceos_search_coceos(h, q.ctypes.data, 2, 16, 5, ids.ctypes.data, ips.ctypes.data)
+        assert rc == 3 and b"not built" in L.ceos_last_error()           # search before build
+        assert L.ceos_search_coceos(h, q.ctypes.data, 2, 15, 5, ids.ctypes.data, ips.ctypes.data) == 1
+    finally:
+        L.ceos_destroy(h)
+    assert L.ceos_create(10, 0, C.byref(h)) == 1
+
+
+def test_module_defaults_mirror_reference(CEOs):
+    # src/CEOs.h:52-55 defaults; src/CEOs.h:69 n_probed_points := top_m
+    g = CEOs.CEOs(1000, 32)
+    assert (g.n_probed_vectors, g.n_probed_points, g.n_cand, g.centering) == (10, 10, 10, False)
+    g.setIndexParam(256, 2, 50, 4, 42)
+    assert g.n_probed_points == 50
+    assert g.fhtDim == 256
+    g.n_probed_vectors, g.n_cand, g.centering = 7, 99, True
+    assert (g.n_probed_vectors, g.n_cand, g.centering) == (7, 99, True)
+    # streamCEOs: setIndexParam sets n_probed_vectors := top_points (src/streamCEOs.h:58)
+    s = CEOs.streamCEOs(32)
+    assert (s.n_probed_vectors, s.n_probed_points, s.n_cand) == (10, 10, 10)
+    s.setIndexParam(128, 1, 33, -1, 1)
+    assert s.n_probed_vectors == 33 and s.n_probed_points == 10
+
+
+def test_module_errors_raise_instead_of_exit(CEOs):
+    g = CEOs.CEOs(100, 8)
+    with pytest.raises(ValueError):
+        g.setIndexParam(0)
+    g.setIndexParam(64, 1, 10, -1, 1)
+    with pytest.raises(RuntimeError):
+        g.search_coCEOs_Est(np.zeros((3, 8), np.float32), 5)             # not built
+    with pytest.raises(ValueError):
+        g.build_coCEOs_Est(np.zeros(8, np.float32))                      # not 2-D
+    with pytest.raises(NotImplementedError):
+        g.build_CEOs_Hash(np.zeros((3, 8), np.float32))
+    s = CEOs.streamCEOs(8)
+    with pytest.raises(NotImplementedError):
+        s.hash_search(np.zeros((3, 8), np.float32), 5)

@@ -1,0 +1,273 @@
+"""CPU tests that pin the oracle before it is trusted as the checker.
+
+Each oracle function is checked against something other than its own formula:
+the reference's own FFHT (compiled from the reference sources, or its committed
+outputs), the host C++ library's RNG, an FP64 Hadamard matrix, closed forms,
+invariants, and brute force on tiny inputs. A planted mistake in any stage (a
+dropped sign, a transposed stage order, an off-by-one probe) fails one of them.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def minstd_bits(nbits, seed):
+    """std::default_random_engine (minstd_rand0) + libstdc++ uniform_int_distribution(0,1)
+    downscale path (/usr/include/c++/11/bits/uniform_int_dist.h:317-326), in Python."""
+    x = seed % 2147483647 or 1
+    out = []
+    while len(out) < nbits:
+        x = (16807 * x) % 2147483647
+        r = x - 1
+        if r >= 2147483644:
+            continue
+        out.append(r // 1073741822)
+    return np.array(out, np.uint8)
+
+
+def hadamard(L):
+    n = 1 << L
+    i = np.arange(n)
+    pc = np.vectorize(lambda v: bin(v).count("1"))(i[:, None] & i[None, :])
+    return np.where(pc % 2 == 0, 1.0, -1.0)
+
+
+# ------------------------------------------------------------------ sign bits
+def test_sign_bits_match_library_and_closed_form(oracle):
+    gold = json.load(open(os.path.join(GOLD, "sign_bits.json")))
+    for seed, s in gold.items():
+        ref = np.frombuffer(s.encode(), np.uint8) - ord("0")
+        got = oracle.sign_bits(len(ref), int(seed))
+        assert np.array_equal(got, ref), seed
+        assert np.array_equal(minstd_bits(len(ref), int(seed)), ref), seed
+
+
+def test_sign_bits_kat_seed42(oracle):
+    # first 64 bits for seed 42 (std::default_random_engine, libstdc++)
+    kat = "0110001110011000000111101010000001111110000100100000110101011010"
+    assert "".join(map(str, oracle.sign_bits(64, 42))) == kat
+
+
+# ------------------------------------------------------------------ FWHT
+@pytest.mark.parametrize("L", range(1, 13))
+def test_fwht_matches_reference_ffht(oracle, L):
+    z = np.load(os.path.join(GOLD, "ffht_vectors.npz"))
+    x, y = z[f"x{L}"], z[f"y{L}"]
+    assert np.array_equal(oracle.fwht_f32(x).view(np.uint32), y.view(np.uint32))
+    ff = oracle.ffht_ref()
+    if ff is not None:  # the reference's fht_float compiled from its own sources
+        rng = np.random.default_rng(L)
+        for _ in range(4):
+            v = rng.standard_normal(1 << L).astype(np.float32)
+            b = v.copy()
+            ff.fht_float(b, L)
+            assert np.array_equal(oracle.fwht_f32(v).view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("L", [1, 3, 6, 9])
+def test_fwht_f64_equals_hadamard_matrix(oracle, L):
+    rng = np.random.default_rng(L)
+    x = rng.standard_normal(1 << L)
+    np.testing.assert_allclose(oracle.fwht_f64(x), hadamard(L) @ x, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("L", [4, 10, 12])
+def test_fwht_invariants(oracle, L):
+    n = 1 << L
+    rng = np.random.default_rng(L)
+    x = rng.standard_normal(n)
+    hx = oracle.fwht_f64(x)
+    np.testing.assert_allclose(oracle.fwht_f64(hx), n * x, rtol=1e-10, atol=1e-9)      # H H = n I
+    np.testing.assert_allclose((hx ** 2).sum(), n * (x ** 2).sum(), rtol=1e-10)         # Parseval
+    e0 = np.zeros(n); e0[0] = 1
+    np.testing.assert_array_equal(oracle.fwht_f64(e0), np.ones(n))                     # H e0 = 1
+    y = rng.standard_normal(n)
+    np.testing.assert_allclose(oracle.fwht_f64(2 * x - 3 * y), 2 * hx - 3 * oracle.fwht_f64(y), rtol=1e-9,
+                               atol=1e-9)  # linearity
+    # FP32 ascending-stage transform stays within the FP64 error bound L * u * ||x||_1-ish
+    x32 = x.astype(np.float32)
+    err = np.abs(oracle.fwht_f32(x32).astype(np.float64) - oracle.fwht_f64(x32.astype(np.float64)))
+    assert err.max() <= L * 2.0 ** -24 * np.abs(x32).sum() * 2
+
+
+# ------------------------------------------------------------------ projection
+def test_projection_is_sign_fwht_composition(oracle):
+    """P = first D of (H S3 H S2 H S1 [x;0]) per repeat, checked in FP64 with the fixture bits."""
+    n, d, D, R = 50, 37, 64, 3
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    o = oracle.OracleCEOs(n, d)
+    o.setIndexParam(D, R, 4, 1, 42)
+    F = o.F
+    assert F == 64
+    bits = oracle.sign_bits(F * 3 * R, 42).astype(np.float64)
+    H = hadamard(6)
+    P = o.project(X).astype(np.float64)
+    for r in range(R):
+        M = np.eye(F)
+        for i in range(3):
+            s = 2 * bits[r * 3 * F + i * F: r * 3 * F + (i + 1) * F] - 1
+            M = H @ (s[:, None] * M)
+        xp = np.zeros((n, F)); xp[:, :d] = X
+        exp = (xp @ M.T)[:, :D]
+        err = np.abs(P[:, r * D:(r + 1) * D] - exp)
+        bound = 3 * 6 * 2.0 ** -24 * np.abs(xp).sum(1, keepdims=True) * F * 2
+        assert (err <= bound).all()
+    # a planted mistake (sign of round 2 dropped) is caught
+    s_wrong = bits.copy(); s_wrong[F:2 * F] = 1
+    M = np.eye(F)
+    for i in range(3):
+        s = 2 * s_wrong[i * F:(i + 1) * F] - 1
+        M = H @ (s[:, None] * M)
+    xp = np.zeros((n, F)); xp[:, :d] = X
+    assert np.abs(P[:, :D] - (xp @ M.T)[:, :D]).max() > 1e-2
+
+
+def test_fht_dim_rule(oracle):
+    # src/CEOs.h:76-79
+    assert oracle.fht_dim(1024, 300) == 1024
+    assert oracle.fht_dim(1024, 960) == 1024
+    assert oracle.fht_dim(512, 384) == 512
+    assert oracle.fht_dim(100, 129) == 256
+    assert oracle.fht_dim(256, 256) == 256
+    assert oracle.fht_dim(257, 10) == 512
+
+
+# ------------------------------------------------------------------ buckets
+def test_buckets_equal_bruteforce_sort(oracle):
+    n, d, D, R, m = 400, 16, 32, 2, 15
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    o = oracle.OracleCEOs(n, d)
+    o.setIndexParam(D, R, m, 2, 42)
+    o.build_coCEOs_Est(X)
+    P = o.project(X)
+    cid, cv, fid, fv = o.buckets()
+    for c in range(D * R):
+        col = P[:, c]
+        order = np.lexsort((np.arange(n), -col))[:m]       # value desc, id asc
+        assert np.array_equal(cv[c], col[order])
+        assert np.array_equal(cid[c], order)
+        order = np.lexsort((np.arange(n), col))[:m]        # far: -value desc
+        assert np.array_equal(fv[c], -col[order])
+        assert np.array_equal(fid[c], order)
+
+
+def test_tiny_index_fixture(oracle):
+    z = np.load(os.path.join(GOLD, "tiny_index.npz"))
+    D, R, m, seed, s, mp, c, k = z["params"].tolist()
+    X, Q = z["X"], z["Q"]
+    o = oracle.OracleCEOs(*X.shape)
+    o.setIndexParam(D, R, m, 1, seed)
+    o.build_coCEOs_Est(X)
+    cid, cv, fid, fv = o.buckets()
+    assert np.array_equal(cid, z["cid"]) and np.array_equal(fid, z["fid"])
+    assert np.array_equal(cv.view(np.uint32), z["cval"].view(np.uint32))
+    o.n_probed_vectors, o.n_cand = s, c
+    ids, ips = o.search_coCEOs_Est(Q, k)
+    assert np.array_equal(ids, z["ids"])
+    assert np.array_equal(ips.view(np.uint32), z["ips"].view(np.uint32))
+
+
+# ------------------------------------------------------------------ search
+def test_full_budget_search_is_exact_mips(oracle):
+    """With every point in every bucket and every point reranked, coCEOs returns exact MIPS."""
+    n, d = 300, 24
+    rng = np.random.default_rng(6)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    Q = rng.standard_normal((20, d)).astype(np.float32)
+    o = oracle.OracleCEOs(n, d)
+    o.setIndexParam(32, 1, n, 1, 42)
+    o.build_coCEOs_Est(X)
+    o.n_probed_vectors, o.n_cand = 5, n
+    ids, ips = o.search_coCEOs_Est(Q, 10)
+    ex, exip = oracle.exact_mips(X, Q, 10)
+    assert np.array_equal(ids, ex)
+    np.testing.assert_allclose(ips, (Q.astype(np.float64) @ X.T.astype(np.float64))[np.arange(20)[:, None], ex],
+                               rtol=1e-6)
+
+
+def test_coceos_with_full_buckets_equals_ceos_est(oracle):
+    """m = n: every bucket holds every point, so coCEOs' estimates are CEOs-Est's (same probes)."""
+    n, d = 500, 30
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal((n, d)).astype(np.float32) * rng.lognormal(0, .5, (n, 1)).astype(np.float32)
+    Q = rng.standard_normal((30, d)).astype(np.float32)
+    a = oracle.OracleCEOs(n, d); a.setIndexParam(64, 2, n, 1, 42); a.build_coCEOs_Est(X)
+    b = oracle.OracleCEOs(n, d); b.setIndexParam(64, 2, n, 1, 42); b.build(X)
+    for o in (a, b):
+        o.n_probed_vectors, o.n_cand = 7, 40
+    ia, pa = a.search_coCEOs_Est(Q, 10)
+    ib, pb = b.search(Q, 10)
+    assert np.array_equal(ia, ib)
+    assert np.array_equal(pa, pb)
+
+
+def test_probe_order_and_scores_bruteforce(oracle):
+    """Estimates recomputed in numpy from projections, probes and buckets pick the same candidates."""
+    n, d, D, R, m, s, c, k = 800, 20, 64, 2, 40, 9, 60, 10
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    Q = rng.standard_normal((25, d)).astype(np.float32)
+    o = oracle.OracleCEOs(n, d)
+    o.setIndexParam(D, R, m, 1, 42)
+    o.build_coCEOs_Est(X)
+    o.n_probed_vectors, o.n_cand = s, c
+    ids, _ = o.search_coCEOs_Est(Q, k)
+    cid, cv, fid, fv = o.buckets()
+    pq = o.project(Q)
+    for q in range(len(Q)):
+        p = pq[q]
+        cols = np.lexsort((np.arange(D * R), -np.abs(p)))[:s]
+        est = {}
+        for col in cols[::-1]:  # ascending |p|: the reference's pop order
+            lids, lv = (cid[col], cv[col]) if p[col] >= 0 else (fid[col], fv[col])
+            for i, v in zip(lids, lv):
+                est[i] = np.float32(est.get(i, np.float32(0)) + v)
+        cand = sorted(est, key=lambda i: (-est[i], i))[:c]
+        ip = X[cand].astype(np.float64) @ Q[q].astype(np.float64)
+        top = [cand[j] for j in np.lexsort((np.array(cand), -ip))[:k]]
+        assert list(ids[q]) == top
+
+
+# ------------------------------------------------------------------ streamCEOs
+def test_stream_update_closed_form(oracle):
+    """bucket' = top_m({old entries with id >= n_del} U {new points}), ids shifted by -n_del."""
+    d, D, R, m = 12, 32, 2, 20
+    rng = np.random.default_rng(9)
+    X0 = rng.standard_normal((300, d)).astype(np.float32)
+    X1 = rng.standard_normal((120, d)).astype(np.float32)
+    n_del = 90
+    st = oracle.OracleStream(d)
+    st.setIndexParam(D, R, m, 1, 42)
+    st.build(X0)
+    cid0, cv0, fid0, fv0, _, _ = st.buckets()
+    st.update(X1, n_del)
+    assert st.size() == 300 + 120 - 90
+    cid, cv, fid, fv, cs, fs = st.buckets()
+    o = oracle.OracleCEOs(120, d)
+    o.setIndexParam(D, R, m, 1, 42)
+    Pn = o.project(X1)
+    for c in range(D * R):
+        for (ids0, v0, ids, v, sgn) in ((cid0, cv0, cid, cv, 1), (fid0, fv0, fid, fv, -1)):
+            keep = ids0[c] >= n_del
+            cand_id = np.concatenate([ids0[c][keep], 300 + np.arange(120)]) - n_del
+            cand_v = np.concatenate([v0[c][keep], sgn * Pn[:, c]])
+            order = np.lexsort((cand_id, -cand_v))[:m]
+            assert np.array_equal(v[c], cand_v[order])
+            assert np.array_equal(ids[c], cand_id[order])
+    assert (cs == m).all() and (fs == m).all()
+
+
+def test_stream_reference_errors(oracle):
+    st = oracle.OracleStream(8)
+    st.setIndexParam(16, 1, 10, 1, 42)
+    st.build(np.ones((20, 8), np.float32))
+    with pytest.raises(ValueError):
+        st.update(np.ones((1, 8), np.float32), 25)      # n_del > size
+    with pytest.raises(ValueError):
+        st.update(np.ones((1, 8), np.float32), 15)      # fewer than top_m left
