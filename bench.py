@@ -82,7 +82,10 @@ def main():
     log(f"generated data in {time.time() - t:.1f}s")
     dX = torch.from_numpy(X).to(dev)
     dQ = torch.from_numpy(Q).to(dev)
-    stream = torch.cuda.current_stream().cuda_stream
+    # a real (non-null) stream: the library launches on it and the events below bracket them
+    ts = torch.cuda.Stream()
+    torch.cuda.set_stream(ts)
+    stream = ts.cuda_stream
     L = capi.lib()
 
     index = CEOs.CEOs(n, d)

@@ -230,9 +230,12 @@ struct SelectShared {
 // (k >> shift) >= (prefix >> shift). With unique keys exactly min(K, n) keys are
 // selected; with repeated keys and min_shift > 0 the prefix holds the K-th largest
 // key's bits above min_shift. Must be called by all threads of the block.
+// Keys are read 8 per thread at a time (independent loads in flight) and equal
+// consecutive digits of one thread are added to the histogram as one atomic.
 template <int NT, typename GetKey>
 __device__ void radix_select(int n, int K, GetKey get, SelectShared& sh, uint64_t& prefix_out, int& shift_out,
                              int min_shift = 0) {
+    constexpr int UNR = 8;
     const int tid = threadIdx.x;
     if (K >= n) {  // everything selected
         prefix_out = 0;
@@ -245,10 +248,29 @@ __device__ void radix_select(int n, int K, GetKey get, SelectShared& sh, uint64_
     for (;;) {
         for (int b = tid; b < 256; b += NT) sh.hist[b] = 0;
         __syncthreads();
-        for (int i = tid; i < n; i += NT) {
-            const uint64_t k = get(i);
-            if ((k & mask) == prefix) atomicAdd(&sh.hist[(uint32_t)(k >> shift) & 255u], 1u);
+        uint32_t run_bin = 0xffffffffu, run_cnt = 0;
+        for (int i0 = tid; i0 < n; i0 += UNR * NT) {
+            uint64_t k[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int i = i0 + u * NT;
+                k[u] = i < n ? get(i) : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int i = i0 + u * NT;
+                if (i < n && (k[u] & mask) == prefix) {
+                    const uint32_t bn = (uint32_t)(k[u] >> shift) & 255u;
+                    if (bn != run_bin) {
+                        if (run_cnt) atomicAdd(&sh.hist[run_bin], run_cnt);
+                        run_bin = bn;
+                        run_cnt = 0;
+                    }
+                    ++run_cnt;
+                }
+            }
         }
+        if (run_cnt) atomicAdd(&sh.hist[run_bin], run_cnt);
         __syncthreads();
         if (tid < 64) {
             const int ln = tid;
@@ -284,6 +306,33 @@ __device__ void radix_select(int n, int K, GetKey get, SelectShared& sh, uint64_
 
 __device__ __forceinline__ bool selected(uint64_t k, uint64_t prefix, int shift) {
     return shift >= 63 ? true : ((k >> shift) >= (prefix >> shift));
+}
+
+// Wave-aggregated append: lanes with `take` get consecutive positions from *counter
+// (one atomic per wave). Must be called by every lane of the wave.
+__device__ __forceinline__ uint32_t wave_append(bool take, uint32_t* counter) {
+    const int ln = threadIdx.x % kWave;
+    const uint64_t m = __ballot(take);
+    uint32_t base = 0;
+    if (ln == 0 && m) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, 0, kWave);
+    return base + (uint32_t)__popcll(m & ((1ull << ln) - 1ull));
+}
+
+// Compacts the keys selected by (prefix, shift) into out[] (order unspecified).
+template <int NT, typename GetKey, typename Emit>
+__device__ void compact_selected(int n, GetKey get, uint64_t prefix, int shift, uint32_t* counter, Emit emit) {
+    for (int i0 = 0; i0 < n; i0 += NT) {
+        const int i = i0 + (int)threadIdx.x;
+        uint64_t k = 0;
+        bool take = false;
+        if (i < n) {
+            k = get(i);
+            take = selected(k, prefix, shift);
+        }
+        const uint32_t pos = wave_append(take, counter);
+        if (take) emit(pos, k);
+    }
 }
 
 // Descending bitonic sort of N (power of two) 64-bit keys in LDS.

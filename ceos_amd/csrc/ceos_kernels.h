@@ -89,7 +89,12 @@ struct QueryConfig {
 };
 
 struct QueryScratch {
-    uint64_t* cand;    // grid * cand_stride
+    float* qproj;      // nq * D*R query projections
+    int32_t* probes;   // nq * s_max probed list ids, ascending |p|
+    uint64_t* cand;    // grid * cand_stride candidate keys
+    uint2* pairs;      // grid * cand_stride (slot, value) per probed bucket position
+    uint4* stash;      // grid * kSearchStashList entries that missed both tile buckets
+    uint32_t* keys;    // grid << table_log2 tile ids during the ordered pass
     int32_t* work;     // work counter (1 int)
     int32_t* status;   // error flags (1 int): bit0 = hash table overflow
 };
@@ -100,7 +105,11 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
                                 const int32_t* bsize, RowRing rows, const QueryBatch& qb,
                                 const QueryConfig& cfg, const QueryScratch& scr, hipStream_t st);
 
-int coceos_search_lds_bytes(const IndexDims& dm, int table_log2, int s_max, int c_max, int k_max);
+int search_lds_bytes(int d, int table_log2, int s_max, int c_max, int k_max);
+int search_ept(int mp);
+constexpr int kSearchThreads = 512;
+constexpr int kMaxProbedPoints = 4096;
+constexpr int kSearchStashList = 2048;  // n_probed_points bound of the device search
 
 // CEOs-Est (dense) search over column-major projections P [DR][n].
 hipError_t launch_est_search(const IndexDims& dm, const uint32_t* signs, const float* P, int64_t n,

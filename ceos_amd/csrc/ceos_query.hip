@@ -1,4 +1,4 @@
-// coCEOs and CEOs-Est query kernels (gfx950).
+// CEOs-Est query kernel (gfx950); the coCEOs search lives in ceos_search.hip.
 //
 // One workgroup (4 waves) serves one query at a time and takes the next query from
 // a global work counter (persistent grid), so queries with different k, s and c in
@@ -87,9 +87,6 @@ __host__ __device__ inline QLayout q_layout(int d, int L, int DR, int tlog2, int
     return o;
 }
 
-int coceos_search_lds_bytes(const IndexDims& dm, int table_log2, int s_max, int c_max, int k_max) {
-    return q_layout(dm.d, dm.L, dm.D * dm.R, table_log2, s_max, c_max, k_max).total;
-}
 int est_search_lds_bytes(const IndexDims& dm, int s_max, int c_max, int k_max) {
     return q_layout(dm.d, dm.L, dm.D * dm.R, 0, s_max, c_max, k_max).total;
 }
@@ -237,112 +234,6 @@ __device__ __forceinline__ int next_query(int32_t* work, int* misc) {
     const int q = misc[0];
     __syncthreads();
     return q;
-}
-
-// ---------------------------------------------------------------------------------
-// coCEOs fused search
-// ---------------------------------------------------------------------------------
-template <int L>
-__global__ __launch_bounds__(kQNT) void k_coceos_search(IndexDims dm, const uint32_t* __restrict__ signs,
-                                                        const Entry* __restrict__ bkt,
-                                                        const int32_t* __restrict__ bsize, RowRing rows,
-                                                        QueryBatch qb, QueryConfig cfg, QueryScratch scr) {
-    __shared__ SelectShared sh;
-    extern __shared__ __align__(16) unsigned char lds[];
-    const int DR = dm.D * dm.R;
-    const QLayout lo = q_layout(dm.d, L, DR, cfg.table_log2, cfg.s_max, cfg.c_max, cfg.k_max);
-    float* qv = reinterpret_cast<float*>(lds + lo.q_off);
-    int* probes = reinterpret_cast<int*>(lds + lo.probe_off);
-    uint64_t* pkey = reinterpret_cast<uint64_t*>(lds + lo.pkey_off);
-    int* misc = reinterpret_cast<int*>(lds + lo.misc_off);
-    float* tile = reinterpret_cast<float*>(lds + lo.tile_off);
-    float* proj = reinterpret_cast<float*>(lds + lo.proj_off);
-    uint32_t* tkeys = reinterpret_cast<uint32_t*>(lds + lo.tab_off);
-    const int T = 1 << cfg.table_log2;
-    float* tvals = reinterpret_cast<float*>(tkeys + T);
-    uint64_t* rk = reinterpret_cast<uint64_t*>(lds + lo.rk_off);
-    uint64_t* tk = reinterpret_cast<uint64_t*>(lds + lo.tk_off);
-    uint64_t* cand = scr.cand + (int64_t)blockIdx.x * cfg.cand_stride;
-    uint64_t* sel = cand + (cfg.cand_stride - cfg.c_max);
-    const int tid = threadIdx.x;
-    const int tshift = 32 - cfg.table_log2;
-
-    for (;;) {
-        const int q = next_query(scr.work, misc);
-        if (q >= qb.nq) break;
-        const int kq = qb.k ? qb.k[q] : qb.k0;
-        const int sq = qb.s ? qb.s[q] : qb.s0;
-        const int cq = qb.c ? qb.c[q] : qb.c0;
-        const float* qrow = qb.Q + (int64_t)q * dm.d;
-
-        project_query<L>(dm, qrow, signs, tile, proj, qv);
-        select_probes(DR, sq, proj, pkey, probes, sh);
-
-        if (tid == 0) {
-            int E = 0;
-            for (int j = 0; j < sq; ++j) E += min(qb.mp, bsize[probes[j]]);
-            misc[1] = E;
-            misc[2] = 0;
-        }
-        __syncthreads();
-        const int E = misc[1];
-        const int half = T >> 1;
-        const uint32_t P = (uint32_t)max(1, (E + half - 1) / half);
-        for (uint32_t pass = 0; pass < P; ++pass) {
-            for (int i = tid; i < T; i += kQNT) tkeys[i] = kEmpty;
-            __syncthreads();
-            for (int j = 0; j < sq; ++j) {
-                const int lj = probes[j];
-                const int lim = min(qb.mp, bsize[lj]);
-                const Entry* bl = bkt + (int64_t)lj * dm.m;
-                for (int t = tid; t < lim; t += kQNT) {
-                    const Entry e = bl[t];
-                    const uint32_t lid = e.x - rows.base;
-                    if (P > 1 && (uint32_t)(((uint64_t)(lid * 0x85ebca77u) * P) >> 32) != pass) continue;
-                    const float v = __uint_as_float(e.y);
-                    uint32_t h = (lid * 0x9e3779b1u) >> tshift;
-                    int probe = 0;
-                    for (; probe < T; ++probe) {
-                        const uint32_t k = tkeys[h];
-                        if (k == lid) { tvals[h] += v; break; }
-                        if (k == kEmpty) {
-                            const uint32_t old = atomicCAS(&tkeys[h], kEmpty, lid);
-                            if (old == kEmpty) { tvals[h] = v; break; }
-                            if (old == lid) { tvals[h] += v; break; }
-                        }
-                        h = (h + 1) & (uint32_t)(T - 1);
-                    }
-                    if (probe == T) atomicOr(scr.status, 1);
-                }
-                __syncthreads();
-            }
-            for (int i = tid; i < T; i += kQNT) {
-                const uint32_t k = tkeys[i];
-                if (k != kEmpty) cand[atomicAdd(&misc[2], 1)] = ckey(tvals[i], k);
-            }
-            __syncthreads();
-        }
-        const int ncand = misc[2];
-        const int ce = select_candidates(cand, ncand, cq, sel, sh);
-        rerank(qv, dm.d, rows, sel, ce, rk);
-        const int64_t off = qb.out_off ? qb.out_off[q] : (int64_t)q * qb.k0;
-        write_topk(rk, ce, kq, tk, sh, qb.out_id + off, qb.out_ip + off);
-    }
-}
-
-hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, const Entry* bkt,
-                                const int32_t* bsize, RowRing rows, const QueryBatch& qb,
-                                const QueryConfig& cfg, const QueryScratch& scr, hipStream_t st) {
-    if (qb.nq <= 0) return hipSuccess;
-    hipError_t err = launch_fill_i32(scr.work, 0, 1, st);
-    if (err != hipSuccess) return err;
-    const int grid = std::max(1, std::min(cfg.grid, qb.nq));
-    CEOS_SWITCH_L(dm.L, {
-        allow_lds(k_coceos_search<LL>, cfg.lds_bytes);
-        hipLaunchKernelGGL(k_coceos_search<LL>, dim3(grid), dim3(kQNT), cfg.lds_bytes, st, dm, signs, bkt,
-                           bsize, rows, qb, cfg, scr);
-    });
-    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------------

@@ -129,7 +129,7 @@ int device_cus() {
 }
 
 constexpr int kMaxLdsBytes = 160 * 1024 - 2048;  // minus the kernels' static LDS
-constexpr int kMaxTopM = 8192;
+constexpr int kMaxTopM = 8192;  // bucket depth; searches read at most kMaxProbedPoints of it
 constexpr int kMaxProbes = 4096;
 constexpr int kMaxK = 4096;
 
@@ -158,6 +158,11 @@ struct Core {
 
     // query scratch
     DBuf<uint64_t> cand;
+    DBuf<float> qproj;
+    DBuf<uint2> pairs;
+    DBuf<uint4> stash;
+    DBuf<uint32_t> tkeys;
+    DBuf<int32_t> probes;
     DBuf<int32_t> work, status;
     int wg_per_cu = 0, table_log2 = 0;
     int last_lds = 0, last_tlog2 = 0, last_wg = 0;
@@ -200,7 +205,7 @@ struct Core {
 
     void clear() {
         bkt.release(); bsize.release(); X.release(); signs.release(); dead.release();
-        cand.release(); work.release(); status.release(); estP.release();
+        cand.release(); qproj.release(); probes.release(); pairs.release(); stash.release(); tkeys.release(); work.release(); status.release(); estP.release();
         bits.clear();
         cap = head = 0; base = live = 0; est_n = 0;
         built = false;
@@ -295,39 +300,49 @@ struct Core {
         HIP_CHECK(hipStreamSynchronize(s_));
     }
 
-    // Picks the LDS score-tile size and grid for a search with the given bounds.
+    // Picks the LDS score-tile size and grid for a search with the given bounds:
+    // the largest tile that fits the per-workgroup LDS share (wg_per_cu workgroups
+    // of 512 threads per CU; default 1, i.e. a 32K-slot tile of 4-byte ids).
     ceos::QueryConfig search_config(int s_max, int c_max, int k_max) {
-        const IndexDims dm = dims();
-        const int wg = wg_per_cu > 0 ? wg_per_cu : 2;
+        const int wg = wg_per_cu > 0 ? wg_per_cu : 1;
         const int budget = std::min(kMaxLdsBytes, (160 * 1024) / wg - 2048);
         int t = 0;
         if (table_log2 > 0) {
             t = table_log2;
         } else {
-            for (int cand_t = 16; cand_t >= 8; --cand_t)
-                if (ceos::coceos_search_lds_bytes(dm, cand_t, s_max, c_max, k_max) <= budget) { t = cand_t; break; }
+            for (int cand_t = 15; cand_t >= 8; --cand_t)
+                if (ceos::search_lds_bytes(d, cand_t, s_max, c_max, k_max) <= budget) { t = cand_t; break; }
             if (t == 0) t = 8;
         }
-        const int lds = ceos::coceos_search_lds_bytes(dm, t, s_max, c_max, k_max);
+        const int lds = ceos::search_lds_bytes(d, t, s_max, c_max, k_max);
         if (lds > kMaxLdsBytes)
             throw ArgError("search parameters need " + std::to_string(lds) +
                            " bytes of LDS per workgroup (n_cand or n_probed_vectors too large)");
+        if (std::min(mp, m) > ceos::kMaxProbedPoints)
+            throw ArgError("n_probed_points above " + std::to_string(ceos::kMaxProbedPoints) + " is not supported");
+        if ((int64_t)s_max * std::min(mp, m) > (int64_t(1) << 26))
+            throw ArgError("n_probed_vectors * n_probed_points above 2^26 is not supported");
         ceos::QueryConfig cfg{};
         cfg.table_log2 = t;
         cfg.lds_bytes = lds;
         const int fit = std::max(1, (160 * 1024) / (lds + 2048));
         cfg.grid = device_cus() * std::min(wg, fit);
         cfg.s_max = s_max; cfg.c_max = c_max; cfg.k_max = k_max;
-        cfg.cand_stride = (int64_t)s_max * std::max(1, std::min(mp, m)) + c_max;
+        cfg.cand_stride = (int64_t)s_max * std::max(1, std::min(mp, m));
         last_lds = lds; last_tlog2 = t; last_wg = std::min(wg, fit);
         return cfg;
     }
 
-    ceos::QueryScratch scratch(const ceos::QueryConfig& cfg) {
+    ceos::QueryScratch scratch(const ceos::QueryConfig& cfg, int nq) {
         cand.ensure((size_t)cfg.grid * cfg.cand_stride);
+        pairs.ensure((size_t)cfg.grid * cfg.cand_stride);
+        stash.ensure((size_t)cfg.grid * ceos::kSearchStashList);
+        tkeys.ensure((size_t)cfg.grid << cfg.table_log2);
+        qproj.ensure((size_t)std::max(nq, 1) * DR());
+        probes.ensure((size_t)std::max(nq, 1) * cfg.s_max);
         if (!work.p) work.alloc(1);
         if (!status.p) { status.alloc(1); HIP_CHECK(hipMemset(status.p, 0, 4)); }
-        return ceos::QueryScratch{cand.p, work.p, status.p};
+        return ceos::QueryScratch{qproj.p, probes.p, cand.p, pairs.p, stash.p, tkeys.p, work.p, status.p};
     }
 
     ceos::RowRing rows() const { return ceos::RowRing{X.p, cap, head, base, live}; }
@@ -355,7 +370,7 @@ struct Core {
     void search_device(const float* dQ, int nq, int k, int s_, int c_, int32_t* ids, float* ips, hipStream_t s__) {
         if (nq <= 0 || k == 0) return;
         const ceos::QueryConfig cfg = search_config(s_, c_, k);
-        const ceos::QueryScratch scr = scratch(cfg);
+        const ceos::QueryScratch scr = scratch(cfg, nq);
         ceos::QueryBatch qb{dQ, nq, nullptr, nullptr, nullptr, k, s_, c_, mp, nullptr, ids, ips};
         HIP_CHECK(ceos::launch_coceos_search(dims(), signs.p, bkt.p, bsize.p, rows(), qb, cfg, scr, s__));
     }
@@ -560,7 +575,7 @@ int ceos_search_batched_device(ceos_index* h, const float* dQ, int nq, int d, co
         if (nq <= 0) return;
         if (!d_k || !d_s || !d_c || !d_off) throw ArgError("per-query arrays must not be null");
         const ceos::QueryConfig cfg = c.search_config(s_max, c_max, std::max(1, k_max));
-        const ceos::QueryScratch scr = c.scratch(cfg);
+        const ceos::QueryScratch scr = c.scratch(cfg, nq);
         ceos::QueryBatch qb{dQ, nq, d_k, d_s, d_c, 0, 0, 0, c.mp, d_off, d_ids, d_ips};
         HIP_CHECK(ceos::launch_coceos_search(c.dims(), c.signs.p, c.bkt.p, c.bsize.p, c.rows(), qb, cfg, scr,
                                              c.stream(stream)));
@@ -599,7 +614,7 @@ int ceos_search_batched(ceos_index* h, const float* Q, int nq, int d, const int3
         HIP_CHECK(hipMemcpyAsync(dc.p, c_, nq * 4, hipMemcpyHostToDevice, c.st()));
         HIP_CHECK(hipMemcpyAsync(doff.p, off.data(), nq * 8, hipMemcpyHostToDevice, c.st()));
         const ceos::QueryConfig cfg = c.search_config(s_max, c_max, std::max(1, k_max));
-        const ceos::QueryScratch scr = c.scratch(cfg);
+        const ceos::QueryScratch scr = c.scratch(cfg, nq);
         ceos::QueryBatch qb{dQ.p, nq, dk.p, ds.p, dc.p, 0, 0, 0, c.mp, doff.p, dI.p, dP.p};
         HIP_CHECK(ceos::launch_coceos_search(c.dims(), c.signs.p, c.bkt.p, c.bsize.p, c.rows(), qb, cfg, scr, c.st()));
         HIP_CHECK(hipMemcpyAsync(ids, dI.p, tot * 4, hipMemcpyDeviceToHost, c.st()));

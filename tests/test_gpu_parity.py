@@ -6,6 +6,7 @@ logged exact ties; top-k ids identical and inner products within 1e-5 relative.
 import numpy as np
 import pytest
 
+from tests._parity import compare_buckets
 from ceos_amd.workloads import WORKLOADS, make_points, make_queries, recall
 
 pytestmark = pytest.mark.gpu
@@ -55,21 +56,7 @@ def _build_pair(CEOs, oracle, X, D, R, m, seed=42):
 
 
 def _compare_buckets(g, o):
-    gc, gcv, gf, gfv = g.get_buckets()
-    oc, ocv, of, ofv = o.buckets()
-    # values must agree bit for bit; ids may differ only where values tie exactly
-    assert np.array_equal(_bits(gcv), _bits(ocv))
-    assert np.array_equal(_bits(gfv), _bits(ofv))
-    mism = (gc != oc).sum() + (gf != of).sum()
-    if mism:
-        # every mismatch must sit inside a run of equal values
-        for ids_g, ids_o, vals in ((gc, oc, ocv), (gf, of, ofv)):
-            rows, cols = np.nonzero(ids_g != ids_o)
-            for r, c in zip(rows, cols):
-                v = vals[r, c]
-                assert (vals[r] == v).sum() > 1, f"list {r} slot {c}: id mismatch without a value tie"
-                assert set(ids_g[r][vals[r] == v]) == set(ids_o[r][vals[r] == v])
-    return mism
+    return compare_buckets(g.get_buckets(), o.buckets())
 
 
 def test_buckets_small(CEOs, oracle):
