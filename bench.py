@@ -18,6 +18,7 @@ Prints one JSON line on stdout; progress goes to stderr.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -53,7 +54,8 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override number of points (testing only)")
     ap.add_argument("--nq", type=int, default=0, help="override number of queries (testing only)")
     ap.add_argument("--plain-queries", type=int, default=300)
-    ap.add_argument("--cpu-queries", type=int, default=3000)
+    ap.add_argument("--cpu-queries", type=int, default=0, help="CPU baseline sample (default: all queries)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="repeat the CPU sample for this long")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -126,6 +128,19 @@ def main():
     ms_step = ev0.elapsed_time(ev1) / args.steps
     qps = nq / (ms_step / 1e3)
     log(f"batched: {ms_step:.3f} ms/step (events), wall {wall / args.steps * 1e3:.3f} ms/step, {qps:,.0f} QPS")
+    # the fused search kernel alone: HIP events recorded by the library on the launch
+    # stream around each kernel, over a second timed run of the same steps
+    capi.check(L.ceos_set_kernel_timing(h, 1))
+    for _ in range(args.steps):
+        step()
+    front = ctypes.c_double()
+    fused = ctypes.c_double()
+    nl = ctypes.c_int()
+    capi.check(L.ceos_get_kernel_timing(h, ctypes.byref(front), ctypes.byref(fused), ctypes.byref(nl)))
+    capi.check(L.ceos_set_kernel_timing(h, 0))
+    fused_ms = fused.value / max(1, nl.value)
+    front_ms = front.value / max(1, nl.value)
+    log(f"kernel events: fused search {fused_ms:.3f} ms/launch, projection+probe select {front_ms:.3f} ms/launch")
     wg, tl, lds = index.get_search_config()
     log(f"search config: {wg} workgroups/CU, LDS tile 2^{tl} slots, {lds} B LDS")
 
@@ -137,21 +152,26 @@ def main():
 
     # plain per-query path: one call per query, synchronised
     P = min(args.plain_queries, nq)
-    dI1 = torch.empty(k, dtype=torch.int32, device=dev)
-    dP1 = torch.empty(k, dtype=torch.float32, device=dev)
-    for i in range(3):
-        capi.check(L.ceos_search_coceos_device(h, dQ[i].data_ptr(), 1, d, k, dI1.data_ptr(), dP1.data_ptr(), stream))
-    torch.cuda.synchronize()
-    lat = []
-    t0 = time.perf_counter()
-    for i in range(P):
-        a = time.perf_counter()
-        capi.check(L.ceos_search_coceos_device(h, dQ[i].data_ptr(), 1, d, k, dI1.data_ptr(), dP1.data_ptr(), stream))
+    plain_qps, lat = None, np.zeros(1)
+    if P > 0:
+        dI1 = torch.empty(k, dtype=torch.int32, device=dev)
+        dP1 = torch.empty(k, dtype=torch.float32, device=dev)
+        for i in range(3):
+            capi.check(L.ceos_search_coceos_device(h, dQ[i].data_ptr(), 1, d, k, dI1.data_ptr(), dP1.data_ptr(),
+                                                   stream))
         torch.cuda.synchronize()
-        lat.append(time.perf_counter() - a)
-    plain_qps = P / (time.perf_counter() - t0)
-    lat = np.array(lat) * 1e3
-    log(f"plain per-query: {plain_qps:,.0f} QPS, p50 {np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms")
+        lat = []
+        t0 = time.perf_counter()
+        for i in range(P):
+            a = time.perf_counter()
+            capi.check(L.ceos_search_coceos_device(h, dQ[i].data_ptr(), 1, d, k, dI1.data_ptr(), dP1.data_ptr(),
+                                                   stream))
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - a)
+        plain_qps = P / (time.perf_counter() - t0)
+        lat = np.array(lat) * 1e3
+        log(f"plain per-query: {plain_qps:,.0f} QPS, p50 {np.percentile(lat, 50):.3f} ms "
+            f"p99 {np.percentile(lat, 99):.3f} ms")
 
     # CPU baseline (oracle restatement of the reference search) on a bounded sample
     cpu = None
@@ -165,28 +185,36 @@ def main():
         o.setIndexParam(w.D, w.R, w.m, threads, w.seed)
         o.load_index(X, cid, cv, fid, fv)
         o.n_probed_vectors, o.n_probed_points, o.n_cand = w.s, w.m, w.c
-        S = min(args.cpu_queries, nq)
+        S = min(args.cpu_queries or nq, nq)
         o.search_coCEOs_Est(Q[:64], k)  # warm caches / thread pool
+        # repeat the sample until about --cpu-seconds of host work is timed
         t0 = time.perf_counter()
         oi, op, distinct, _ = o.search_coCEOs_Est(Q[:S], k, stats=True)
+        reps = 1
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            o.search_coCEOs_Est(Q[:S], k)
+            reps += 1
         cpu_s = time.perf_counter() - t0
-        cpu_qps = S / cpu_s
+        cpu_qps = S * reps / cpu_s
         match = float(np.mean(np.all(oi == ids[:S], axis=1)))
         c_eff = float(np.mean(np.minimum(w.c, distinct)))
-        log(f"cpu oracle: {cpu_qps:,.0f} QPS on {S} queries, {threads} threads; top-k match {match:.4f}; "
+        log(f"cpu oracle: {cpu_qps:,.0f} QPS on {S} queries x {reps}, {threads} threads; top-k match {match:.4f}; "
             f"mean distinct candidates {distinct.mean():.0f}")
         cpu = {"value": round(cpu_qps, 1), "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"first {S} of the {nq} queries on the same index (buckets from the device build, "
-                         f"verified identical to the oracle's build in tests); {cpu_s:.1f} s",
+               "sample": f"{S} of the {nq} queries, {reps} passes, on the same index (buckets from the device "
+                         f"build, verified identical to the oracle's build in tests); {cpu_s:.1f} s",
                "topk_match_vs_gpu": match}
 
-    # roofline of the fused search kernel: algorithmic bytes per query
+    # roofline of the fused search kernel: algorithmic bytes per query are the query
+    # row, the probed bucket entries (8 B each), the candidates' rows and the top-k
+    # output; one launch covers all nq queries
     mp = w.m
     bytes_q = 4 * d + 8 * w.s * mp + 4 * d * c_eff + 8 * k
-    achieved = bytes_q * nq / (ms_step / 1e3) / 1e9
+    achieved = bytes_q * nq / (fused_ms / 1e3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_coceos_search", "bytes_per_query": round(bytes_q)}
+            "kernel": "k_search_fused", "kernel_ms": round(fused_ms, 4), "bytes_per_query": round(bytes_q),
+            "front_kernels_ms": round(front_ms, 4)}
 
     out = {
         "metric": METRIC,
@@ -205,8 +233,8 @@ def main():
                    "top_m": w.m, "n_probed_vectors": w.s, "n_probed_points": mp, "n_cand": w.c, "k": k,
                    "batch": "all queries in one fused launch"},
         f"recall_at_{k}": round(rec, 4),
-        "plain_qps": round(plain_qps, 1),
-        "batched_vs_plain": round(qps / plain_qps, 1),
+        "plain_qps": round(plain_qps, 1) if plain_qps else None,
+        "batched_vs_plain": round(qps / plain_qps, 1) if plain_qps else None,
         "plain_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                              "p99": round(float(np.percentile(lat, 99)), 4)},
         "build_mpts_s": round(n / build_s / 1e6, 2),

@@ -44,6 +44,8 @@ def lib():
             "ceos_set_search_config": (i32, [vp, i32, i32]),
             "ceos_get_search_config": (i32, [vp, P, P, P]),
             "ceos_get_stream": (vp, [vp]),
+            "ceos_set_kernel_timing": (i32, [vp, i32]),
+            "ceos_get_kernel_timing": (i32, [vp, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ceos {
 
 constexpr int kWave = 64;
@@ -226,13 +228,15 @@ struct SelectShared {
     uint32_t counter;
 };
 
-// Exact top-K (K >= 1) of n keys given by get(i). On return, key k is selected iff
+// Exact top-K (K >= 1) of n keys given by get(i), or by get(i, shift) where only the
+// key bits at and above `shift` need to be valid. On return, key k is selected iff
 // (k >> shift) >= (prefix >> shift). With unique keys exactly min(K, n) keys are
 // selected; with repeated keys and min_shift > 0 the prefix holds the K-th largest
-// key's bits above min_shift. Must be called by all threads of the block.
+// key's bits above min_shift. With kSkipZero, zero keys are absent (not counted, never
+// selected; K must not exceed the number of nonzero keys). Must be called by all threads of the block.
 // Keys are read 8 per thread at a time (independent loads in flight) and equal
 // consecutive digits of one thread are added to the histogram as one atomic.
-template <int NT, typename GetKey>
+template <int NT, typename GetKey, bool kSkipZero = false>
 __device__ void radix_select(int n, int K, GetKey get, SelectShared& sh, uint64_t& prefix_out, int& shift_out,
                              int min_shift = 0) {
     constexpr int UNR = 8;
@@ -254,12 +258,15 @@ __device__ void radix_select(int n, int K, GetKey get, SelectShared& sh, uint64_
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
                 const int i = i0 + u * NT;
-                k[u] = i < n ? get(i) : 0ull;
+                if constexpr (std::is_invocable_v<GetKey, int, int>)
+                    k[u] = i < n ? get(i, shift) : 0ull;  // getter told which bits are needed
+                else
+                    k[u] = i < n ? get(i) : 0ull;
             }
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
                 const int i = i0 + u * NT;
-                if (i < n && (k[u] & mask) == prefix) {
+                if (i < n && (k[u] & mask) == prefix && (!kSkipZero || k[u] != 0ull)) {
                     const uint32_t bn = (uint32_t)(k[u] >> shift) & 255u;
                     if (bn != run_bin) {
                         if (run_cnt) atomicAdd(&sh.hist[run_bin], run_cnt);
@@ -308,6 +315,11 @@ __device__ __forceinline__ bool selected(uint64_t k, uint64_t prefix, int shift)
     return shift >= 63 ? true : ((k >> shift) >= (prefix >> shift));
 }
 
+// Number of set bits of a wave mask in the lanes below this one (v_mbcnt).
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Wave-aggregated append: lanes with `take` get consecutive positions from *counter
 // (one atomic per wave). Must be called by every lane of the wave.
 __device__ __forceinline__ uint32_t wave_append(bool take, uint32_t* counter) {
@@ -315,8 +327,28 @@ __device__ __forceinline__ uint32_t wave_append(bool take, uint32_t* counter) {
     const uint64_t m = __ballot(take);
     uint32_t base = 0;
     if (ln == 0 && m) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, 0, kWave);
-    return base + (uint32_t)__popcll(m & ((1ull << ln) - 1ull));
+    base = __builtin_amdgcn_readfirstlane(base);
+    return base + lanes_below(m);
+}
+
+// Reserves cnt (< 2^NB) consecutive positions per lane at *counter (one atomic per
+// wave); returns the lane's first position. Lanes take their ranges in lane order.
+// The lane prefix sum is formed bit plane by bit plane from ballots. Must be called
+// by every lane of the wave.
+template <int NB>
+__device__ __forceinline__ uint32_t wave_append_n(uint32_t cnt, uint32_t* counter) {
+    const int ln = threadIdx.x % kWave;
+    uint32_t excl = 0, total = 0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const uint64_t m = __ballot((cnt >> b) & 1u);
+        excl += lanes_below(m) << b;
+        total += (uint32_t)__popcll(m) << b;
+    }
+    uint32_t base = 0;
+    if (ln == 0 && total) base = atomicAdd(counter, total);
+    base = __builtin_amdgcn_readfirstlane(base);
+    return base + excl;
 }
 
 // Compacts the keys selected by (prefix, shift) into out[] (order unspecified).

@@ -86,6 +86,9 @@ struct QueryConfig {
     int lds_bytes;     // dynamic LDS per workgroup
     int s_max, c_max, k_max;
     int64_t cand_stride;  // scratch entries per workgroup
+    // optional kernel timing: recorded before the query projection and before and
+    // after the fused search kernel (all null: no events)
+    hipEvent_t ev[3];
 };
 
 struct QueryScratch {
@@ -109,7 +112,7 @@ int search_lds_bytes(int d, int table_log2, int s_max, int c_max, int k_max);
 int search_ept(int mp);
 constexpr int kSearchThreads = 512;
 constexpr int kMaxProbedPoints = 4096;
-constexpr int kSearchStashList = 2048;  // n_probed_points bound of the device search
+constexpr int kSearchStashList = 4096;  // stash-list entries per workgroup and pass
 
 // CEOs-Est (dense) search over column-major projections P [DR][n].
 hipError_t launch_est_search(const IndexDims& dm, const uint32_t* signs, const float* P, int64_t n,

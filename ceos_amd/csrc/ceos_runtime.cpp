@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -166,6 +167,10 @@ struct Core {
     DBuf<int32_t> work, status;
     int wg_per_cu = 0, table_log2 = 0;
     int last_lds = 0, last_tlog2 = 0, last_wg = 0;
+    // optional per-launch kernel timing (HIP events on the launch stream)
+    bool ktiming = false;
+    std::vector<std::array<hipEvent_t, 3>> kev;
+    size_t kev_used = 0;
 
     // CEOs-Est projections [DR][n]
     DBuf<float> estP;
@@ -175,7 +180,22 @@ struct Core {
 
     Core() = default;
     ~Core() {
+        for (auto& e : kev)
+            for (hipEvent_t x : e) (void)hipEventDestroy(x);
         if (st_) (void)hipStreamDestroy(st_);
+    }
+    static constexpr size_t kMaxTimedLaunches = 4096;
+    // Attaches the next event triple to a search launch when timing is on.
+    void arm_timing(ceos::QueryConfig& cfg) {
+        cfg.ev[0] = cfg.ev[1] = cfg.ev[2] = nullptr;
+        if (!ktiming || kev_used >= kMaxTimedLaunches) return;
+        if (kev_used == kev.size()) {
+            std::array<hipEvent_t, 3> e{};
+            for (auto& x : e) HIP_CHECK(hipEventCreate(&x));
+            kev.push_back(e);
+        }
+        for (int i = 0; i < 3; ++i) cfg.ev[i] = kev[kev_used][i];
+        ++kev_used;
     }
     // The stream is created on first device use, so handles can be made and
     // configured on a host without a GPU.
@@ -330,6 +350,7 @@ struct Core {
         cfg.s_max = s_max; cfg.c_max = c_max; cfg.k_max = k_max;
         cfg.cand_stride = (int64_t)s_max * std::max(1, std::min(mp, m));
         last_lds = lds; last_tlog2 = t; last_wg = std::min(wg, fit);
+        arm_timing(cfg);
         return cfg;
     }
 
@@ -715,7 +736,7 @@ int ceos_set_search_config(ceos_index* h, int wg_per_cu, int table_log2) {
     return guarded([&] {
         if (!h) throw ArgError("null handle");
         if (wg_per_cu < 0 || wg_per_cu > 8) throw ArgError("wg_per_cu must be in [0, 8]");
-        if (table_log2 != 0 && (table_log2 < 8 || table_log2 > 16)) throw ArgError("table_log2 must be 0 or in [8, 16]");
+        if (table_log2 != 0 && (table_log2 < 8 || table_log2 > 15)) throw ArgError("table_log2 must be 0 or in [8, 15]");
         std::lock_guard<std::mutex> g(h->core.mu);
         h->core.wg_per_cu = wg_per_cu;
         h->core.table_log2 = table_log2;
@@ -728,6 +749,35 @@ int ceos_get_search_config(const ceos_index* h, int* wg_per_cu, int* table_log2,
         if (wg_per_cu) *wg_per_cu = h->core.last_wg;
         if (table_log2) *table_log2 = h->core.last_tlog2;
         if (lds_bytes) *lds_bytes = h->core.last_lds;
+    });
+}
+
+int ceos_set_kernel_timing(ceos_index* h, int on) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.ktiming = on != 0;
+        h->core.kev_used = 0;
+    });
+}
+
+int ceos_get_kernel_timing(ceos_index* h, double* front_ms, double* fused_ms, int* launches) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        double a = 0.0, b = 0.0;
+        for (size_t i = 0; i < c.kev_used; ++i) {
+            HIP_CHECK(hipEventSynchronize(c.kev[i][2]));
+            float x = 0.0f, y = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&x, c.kev[i][0], c.kev[i][1]));
+            HIP_CHECK(hipEventElapsedTime(&y, c.kev[i][1], c.kev[i][2]));
+            a += x;
+            b += y;
+        }
+        if (front_ms) *front_ms = a;
+        if (fused_ms) *fused_ms = b;
+        if (launches) *launches = (int)c.kev_used;
     });
 }
 

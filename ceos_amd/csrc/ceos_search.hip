@@ -7,18 +7,24 @@
 //    list ids 2c (close, p >= 0) / 2c+1 (far, p < 0) (src/CEOs.cpp:517-548, 562-564).
 // 3. k_search_fused (persistent, one 512-thread workgroup per query at a time, next
 //    query from a global work counter so uneven k / s / c balance across the chip):
-//    a. score accumulation into an LDS hash tile of 8-byte (id, FP32 sum) slots.
-//       Bucket entries are prefetched into registers a chunk of probes at a time,
-//       double buffered, and applied probe by probe with a barrier between probes.
-//       Ids within one bucket list are distinct, so each point's sum is formed in
+//    a. score accumulation in two phases over an LDS tile of 2^table_log2 slots.
+//       Phase 1 gives every distinct id of the probed lists a slot (two-choice
+//       buckets of four 4-byte ids, claims by atomicMin between barriers, a small
+//       linear-probing stash for the rare leftovers) and records each bucket
+//       position's slot. Phase 2 turns the tile into FP32 sums and adds the values
+//       probe by probe, a barrier between probes, so each point's sum is formed in
 //       probe order exactly as the reference's robin-map accumulation
-//       (src/CEOs.cpp:558-610). When the distinct ids exceed the tile, ids are split
-//       into hash partitions done in successive passes; the partition count adapts
-//       per query and restarts doubled on overflow;
+//       (src/CEOs.cpp:558-610). Bucket entries are prefetched into registers a
+//       chunk of probes at a time, double buffered. When the distinct ids exceed
+//       the tile, ids are split into hash partitions done in successive passes;
+//       the partition count adapts per workgroup and doubles on overflow;
 //    b. exact top-c of the estimates by (score desc, id asc) (src/CEOs.cpp:619-632);
-//       with fewer distinct candidates than c all are kept;
-//    c. rerank: FP32 inner products of the candidates' rows with the query, eight
-//       rows in flight per lane group (src/CEOs.cpp:638-657);
+//       with fewer distinct candidates than c all are kept. In a single pass the
+//       select reads the tile itself: a value histogram over equal-width bins finds
+//       the boundary bin, which a radix select resolves exactly; with several passes
+//       the (score, id) keys of all passes are selected from scratch;
+//    c. rerank: FP32 inner products of the candidates' rows with the query, 16 rows
+//       in flight per lane group (8 for the largest bucket depths) (src/CEOs.cpp:638-657);
 //    d. top-k by (inner product desc, id asc), written descending, zero padded
 //       (src/CEOs.cpp:659-666).
 #include <algorithm>
@@ -32,14 +38,14 @@ constexpr int kSNT = kSearchThreads;   // 512
 constexpr int kSWaves = kSNT / kWave;
 constexpr int kPNT = 256;              // probe-select block
 constexpr uint32_t kEmptyKey = 0xffffffffu;
-constexpr uint64_t kEmptySlot = 0xffffffffull;  // key = empty, value bits = 0
 constexpr int kBufEntries = 8;         // bucket entries per thread per chunk
-constexpr uint32_t kMulti = 0x80000000u;  // key flag: the id occurs in more than one probed list
-constexpr uint32_t kUnset = 0xffffffffu;  // value bits of a multi slot before its first ordered add
-constexpr uint64_t kNeverSlot = 0x7fffffffull;  // key 0x7fffffff: no slot ever holds it (ids < 2^31-1)
+constexpr uint32_t kUnset = 0xffffffffu;       // sum bits of a slot before its first (ordered) add
 constexpr uint32_t kStashMark = 0xfffffffeu;   // phase-1 slot value: entry sent to the stash
-constexpr int kStashSlots = 1024;              // linear-probing stash at the end of the tile
-constexpr int kMaxRounds = 12;                 // claim rounds per chunk before the stash
+constexpr uint32_t kToPlace = 0xfffffffdu;     // phase-1 slot value: entry not placed yet
+constexpr int kStashSlots = 2048;              // linear-probing stash at the end of the tile
+constexpr int kEmitU = 8;                      // tile slots per thread per emission step
+constexpr int kSelBins = 4096;                 // candidate select: value histogram bins
+constexpr int kBoundMax = 1024;                // boundary-bin keys selected in LDS (else in scratch)
 constexpr int kStashList = kSearchStashList;   // pending stash entries per pass (global list)
 
 __device__ __forceinline__ uint32_t bucket1(uint32_t id, uint32_t nbk) {
@@ -49,7 +55,6 @@ __device__ __forceinline__ uint32_t bucket2(uint32_t id, uint32_t b1, uint32_t n
     const uint32_t b2 = (uint32_t)(((uint64_t)((id ^ 0x5bd1e995u) * 0x85ebca6bu) * nbk) >> 32);
     return b2 == b1 ? (b1 + 1 == nbk ? 0u : b1 + 1) : b2;
 }
-constexpr int kRerankU = 8;            // rows in flight per lane group
 
 template <typename K>
 static void allow_lds_s(K kernel, size_t bytes) {
@@ -61,7 +66,7 @@ static void allow_lds_s(K kernel, size_t bytes) {
 __host__ __device__ inline int al16(int b) { return (b + 15) & ~15; }
 
 struct SLayout {
-    int q_off, probe_off, lim_off, misc_off, cid_off, u_off;
+    int q_off, probe_off, lim_off, misc_off, cid_off, hist_off, u_off;
     int tab_off, rk_off, tk_off, total;
 };
 
@@ -73,6 +78,7 @@ __host__ __device__ inline SLayout s_layout(int d, int tlog2, int s_max, int c_m
     o.lim_off = off; off += al16(s_max * 4);
     o.misc_off = off; off += 64;
     o.cid_off = off; off += al16(c_max * 4);
+    o.hist_off = off; off += kSelBins * 4;
     o.u_off = off;
     o.tab_off = off;
     const int endA = off + (4 << tlog2);
@@ -129,6 +135,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                                                        const int32_t* __restrict__ bsize, RowRing rows,
                                                        QueryBatch qb, QueryConfig cfg, QueryScratch scr) {
     constexpr int CH = kBufEntries / EPT;  // probes per prefetch chunk
+    constexpr int kRerankU = EPT >= 8 ? 8 : 16;  // rows in flight per lane group (register budget)
     constexpr int B = kBufEntries;         // entries per thread per chunk
     __shared__ SelectShared sh;
     extern __shared__ __align__(16) unsigned char lds[];
@@ -172,13 +179,9 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         __syncthreads();
 
         // ---- a. score accumulation ----
-        // Phase 1 (any order): every id of the probed lists gets a tile slot. An id claims an
-        //   empty slot of its bucket with a non-returning atomicMin between barriers; the next
-        //   round's read of the bucket shows whether it holds the slot (reads and claims of a
-        //   round are separated by a barrier, so a read never sees a claim of its own round).
-        //   An id lives in the first of its two buckets that had room when it arrived, and
-        //   nothing is removed within a pass. Ids that find both buckets full go to the stash.
-        //   Each bucket position's (slot, value) is kept in per-workgroup scratch.
+        // Phase 1 (any order): every id of the probed lists gets a tile slot; nothing is
+        //   removed within a pass, so an id found once keeps its slot. Each bucket
+        //   position's (slot, value) is kept in per-workgroup scratch.
         // Phase 2 (probe order, one barrier per probe): the ids move to scratch and the tile
         //   holds FP32 sums; every position adds its value to its slot in probe order, the
         //   first value assigned, which is the reference's per-id summation order
@@ -211,60 +214,81 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 for (int e = 0; e < EPT; ++e) buf[jj][e] = pl[min(tid + e * kSNT, mp - 1)];
             }
         };
+        // One chunk of phase 1, two barriers. Every entry reads both of its buckets; an id
+        // already present takes its slot; otherwise it claims an empty slot of the first
+        // bucket with room (atomicMin, after a barrier so that every read of the chunk has seen
+        // the same table); after a second barrier each claim is read back, and the winners
+        // take their slot. Losers and ids whose buckets are both full go to the stash
+        // list. Copies of one id in a chunk decide alike (same table, same rule), so they
+        // claim the same slot and win or lose together. All reads of a chunk are issued
+        // before any of them is used.
         auto phase1_chunk = [&](const Entry (&buf)[CH][EPT], int j0, uint32_t pass) {
-            uint32_t id[B], bk[B], slot[B];
+            uint32_t id[B], slot[B], b1[B], b2[B];
             float v[B];
-            uint32_t pend = 0, second = 0;
 #pragma unroll
             for (int b = 0; b < B; ++b) {
                 const int j = j0 + b / EPT, t = tid + (b % EPT) * kSNT;
                 id[b] = buf[b / EPT][b % EPT].x - rows.base;
                 v[b] = __uint_as_float(buf[b / EPT][b % EPT].y);
-                bk[b] = bucket1(id[b], nbk);
-                slot[b] = kEmptyKey;
                 const bool valid = j < sq && t < lims[min(j, jlast)] &&
                                    (P == 1 || (uint32_t)(((uint64_t)(id[b] * 0x85ebca77u) * P) >> 32) == pass);
-                pend |= (valid ? 1u : 0u) << b;
+                slot[b] = valid ? kToPlace : kEmptyKey;  // kEmptyKey: not in this pass
+                b1[b] = bucket1(id[b], nbk);
+                b2[b] = bucket2(id[b], b1[b], nbk);
             }
-            for (int round = 0;; ++round) {
-                uint4 kk[B];
+            uint4 k1[B], k2[B];
 #pragma unroll
-                for (int b = 0; b < B; ++b) kk[b] = reinterpret_cast<const uint4*>(keys)[bk[b]];
-                uint32_t claim = 0, cslot[B];
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    const uint32_t k0 = kk[b].x, k1 = kk[b].y, k2 = kk[b].z, k3 = kk[b].w;
-                    const int hit = k0 == id[b] ? 0 : k1 == id[b] ? 1 : k2 == id[b] ? 2 : k3 == id[b] ? 3 : -1;
-                    const int emp = k0 == kEmptyKey ? 0 : k1 == kEmptyKey ? 1 : k2 == kEmptyKey ? 2 : k3 == kEmptyKey ? 3 : -1;
-                    cslot[b] = 4 * bk[b] + (uint32_t)(emp < 0 ? 0 : emp);
-                    if ((pend >> b) & 1u) {
-                        if (hit >= 0) {
-                            slot[b] = 4 * bk[b] + (uint32_t)hit;
-                            pend &= ~(1u << b);
-                        } else if (emp >= 0 && round < kMaxRounds) {
-                            claim |= 1u << b;
-                        } else if (!((second >> b) & 1u) && round < kMaxRounds) {
-                            second |= 1u << b;
-                            bk[b] = bucket2(id[b], bk[b], nbk);
-                        } else {
-                            slot[b] = kStashMark;  // both buckets full (or out of rounds): stash
-                            pend &= ~(1u << b);
-                        }
-                    }
-                }
-                __syncthreads();  // every read of this round precedes every claim of this round
-#pragma unroll
-                for (int b = 0; b < B; ++b)
-                    if ((claim >> b) & 1u) atomicMin(&keys[cslot[b]], id[b]);
-                if (!__syncthreads_or(pend != 0u)) break;
+            for (int b = 0; b < B; ++b) {
+                k1[b] = reinterpret_cast<const uint4*>(keys)[b1[b]];
+                k2[b] = reinterpret_cast<const uint4*>(keys)[b2[b]];
             }
+            uint32_t claim = 0;
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const uint32_t x = id[b];
+                const uint4 p = k1[b], q2 = k2[b];
+                // one-hot match masks (ids within a bucket are distinct)
+                const uint32_t m1 = (p.x == x ? 1u : 0u) | (p.y == x ? 2u : 0u) | (p.z == x ? 4u : 0u) |
+                                    (p.w == x ? 8u : 0u);
+                const uint32_t m2 = (q2.x == x ? 1u : 0u) | (q2.y == x ? 2u : 0u) | (q2.z == x ? 4u : 0u) |
+                                    (q2.w == x ? 8u : 0u);
+                // empty masks: ids are below 2^31, the empty key has the top bit set
+                const uint32_t e1 = (p.x >> 31) | ((p.y >> 30) & 2u) | ((p.z >> 29) & 4u) | ((p.w >> 28) & 8u);
+                const uint32_t e2 = (q2.x >> 31) | ((q2.y >> 30) & 2u) | ((q2.z >> 29) & 4u) | ((q2.w >> 28) & 8u);
+                const bool in1 = m1 != 0u || (m2 == 0u && e1 != 0u);  // target bucket: b1 or b2
+                const uint32_t hm = in1 ? m1 : m2;
+                // a new id takes the first empty slot at or after a per-id start position
+                // (cyclically), so that different ids rarely claim the same slot
+                const uint32_t r = (x * 0x2c1b3c6du) >> 30;
+                const uint32_t em = in1 ? e1 : e2;
+                const uint32_t rot = ((em | (em << 4)) >> r) & 15u;
+                const uint32_t within = hm != 0u ? (uint32_t)__builtin_ctz(hm | 16u)
+                                                 : (((uint32_t)__builtin_ctz(rot | 16u) + r) & 3u);
+                const uint32_t pos = 4u * (in1 ? b1[b] : b2[b]) + within;
+                const bool hit = (m1 | m2) != 0u;
+                const bool room = (e1 | e2) != 0u;
+                const bool place = slot[b] == kToPlace;
+                slot[b] = place ? (hit || room ? pos : kStashMark) : slot[b];
+                claim |= (place && !hit && room ? 1u : 0u) << b;
+            }
+            __syncthreads();  // every read of this chunk precedes every claim of this chunk
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+                if ((claim >> b) & 1u) atomicMin(&keys[slot[b]], id[b]);
+            __syncthreads();  // claims are complete before they are read back
+            uint32_t got[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) got[b] = keys[((claim >> b) & 1u) ? slot[b] : 0u];
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+                if (((claim >> b) & 1u) && got[b] != id[b]) slot[b] = kStashMark;  // lost the claim
 #pragma unroll
             for (int b = 0; b < B; ++b) {
                 const int j = j0 + b / EPT, t = tid + (b % EPT) * kSNT;
                 if (slot[b] == kStashMark) {
-                    const uint32_t pos = atomicAdd(reinterpret_cast<uint32_t*>(&misc[5]), 1u);
-                    if (pos < (uint32_t)kStashList)
-                        stash_list[pos] = make_uint4(id[b], __float_as_uint(v[b]), (uint32_t)(j * mp + t), 0u);
+                    const uint32_t sp = atomicAdd(reinterpret_cast<uint32_t*>(&misc[5]), 1u);
+                    if (sp < (uint32_t)kStashList)
+                        stash_list[sp] = make_uint4(id[b], __float_as_uint(v[b]), (uint32_t)(j * mp + t), 0u);
                     else
                         misc[4] = 1;  // stash list full: the pass restarts with more partitions
                 }
@@ -280,12 +304,15 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             for (int i = tid; i < ns; i += kSNT) {
                 const uint4 e = stash_list[i];
                 const uint32_t idv = e.x;
-                uint32_t found = kEmptyKey;
                 const uint32_t b1 = bucket1(idv, nbk), b2 = bucket2(idv, b1, nbk);
-                for (int q2 = 0; q2 < 8 && found == kEmptyKey; ++q2) {
-                    const uint32_t si = 4 * (q2 < 4 ? b1 : b2) + (q2 & 3);
-                    if (keys[si] == idv) found = si;
-                }
+                const uint4 p = reinterpret_cast<const uint4*>(keys)[b1];
+                const uint4 q2 = reinterpret_cast<const uint4*>(keys)[b2];
+                const uint32_t m1 = (p.x == idv ? 1u : 0u) | (p.y == idv ? 2u : 0u) | (p.z == idv ? 4u : 0u) |
+                                    (p.w == idv ? 8u : 0u);
+                const uint32_t m2 = (q2.x == idv ? 1u : 0u) | (q2.y == idv ? 2u : 0u) | (q2.z == idv ? 4u : 0u) |
+                                    (q2.w == idv ? 8u : 0u);
+                uint32_t found = m1 ? 4u * b1 + (uint32_t)__builtin_ctz(m1)
+                                    : m2 ? 4u * b2 + (uint32_t)__builtin_ctz(m2) : kEmptyKey;
                 if (found == kEmptyKey) {
                     uint32_t h = (idv * 0x9e3779b1u) & (uint32_t)(sslots - 1);
                     for (int probe = 0; probe < sslots; ++probe) {
@@ -304,6 +331,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 if (found != kEmptyKey) pairs[e.z].x = found;
             }
         };
+        uint32_t kmin = 0xffffffffu, kmax = 0u;  // fkey range of every sum written in phase 2
         auto phase2_chunk = [&](const uint2 (&buf)[CH][EPT], int j0) {
 #pragma unroll
             for (int jj = 0; jj < CH; ++jj) {
@@ -324,6 +352,9 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                         const float v = __uint_as_float(buf[jj][e].y);
                         const float nv = cur[e] == kUnset ? v : __uint_as_float(cur[e]) + v;
                         keys[buf[jj][e].x] = __float_as_uint(nv);
+                        const uint32_t fk = fkey(nv);
+                        kmin = min(kmin, fk);
+                        kmax = max(kmax, fk);
                     }
                 }
                 __syncthreads();
@@ -336,7 +367,12 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             bool overflow = false;
             for (uint32_t pass = 0; pass < P && !overflow; ++pass) {
                 for (int i = tid; i < T; i += kSNT) keys[i] = kEmptyKey;
-                if (tid == 0) misc[5] = 0;
+                if (tid == 0) {
+                    misc[5] = 0;
+                    misc[6] = 0;
+                    misc[7] = -1;  // min fkey of the phase-2 sums (as unsigned)
+                    misc[8] = 0;   // max fkey
+                }
                 __syncthreads();
                 // phase 1
                 {
@@ -357,9 +393,16 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 }
                 if (misc[4] != 0) { overflow = true; break; }
                 // ids to scratch; the tile becomes the sums ("unset" until the first value)
-                for (int i = tid; i < T; i += kSNT) {
-                    keys_g[i] = keys[i];
-                    keys[i] = kUnset;
+                {
+                    uint32_t occ = 0;
+                    for (int i = tid; i < T; i += kSNT) {
+                        const uint32_t kv = keys[i];
+                        keys_g[i] = kv;
+                        keys[i] = kUnset;
+                        occ += kv != kEmptyKey ? 1u : 0u;
+                    }
+                    occ = wave_sum(occ);
+                    if (ln == 0) atomicAdd(reinterpret_cast<uint32_t*>(&misc[6]), occ);
                 }
                 __syncthreads();
                 // phase 2
@@ -374,18 +417,41 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                         phase2_chunk(bufB, j0 + CH);
                     }
                 }
+                if (P == 1) {  // single pass: the candidates are selected from the tile itself
+                    // every final sum is one of the sums written, so [kmin, kmax] covers them
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) {
+                        kmin = min(kmin, (uint32_t)__shfl_xor(kmin, o, kWave));
+                        kmax = max(kmax, (uint32_t)__shfl_xor(kmax, o, kWave));
+                    }
+                    if (ln == 0) {
+                        atomicMin(reinterpret_cast<uint32_t*>(&misc[7]), kmin);
+                        atomicMax(reinterpret_cast<uint32_t*>(&misc[8]), kmax);
+                    }
+                    break;
+                }
                 // candidates of this pass: (sum, id) keys to scratch
-                for (int i0 = 0; i0 < T; i0 += kSNT) {  // uniform trip count: wave_append needs all lanes
-                    const int i = i0 + tid;
-                    const uint32_t idv = i < T ? keys_g[i] : kEmptyKey;
-                    const bool take = idv != kEmptyKey;
-                    const uint32_t pos = wave_append(take, reinterpret_cast<uint32_t*>(&misc[2]));
-                    if (take) cand[pos] = ckey(__uint_as_float(keys[i]), idv);
+                for (int i0 = 0; i0 < T; i0 += kEmitU * kSNT) {  // uniform trip count (wave-level append)
+                    uint32_t idv[kEmitU];
+                    uint32_t cnt = 0;
+#pragma unroll
+                    for (int u = 0; u < kEmitU; ++u) {
+                        const int i = i0 + u * kSNT + tid;
+                        idv[u] = i < T ? keys_g[i] : kEmptyKey;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kEmitU; ++u) cnt += idv[u] != kEmptyKey ? 1u : 0u;
+                    uint32_t pos = wave_append_n<4>(cnt, reinterpret_cast<uint32_t*>(&misc[2]));
+#pragma unroll
+                    for (int u = 0; u < kEmitU; ++u) {
+                        const int i = i0 + u * kSNT + tid;
+                        if (idv[u] != kEmptyKey) cand[pos++] = ckey(__uint_as_float(keys[i]), idv[u]);
+                    }
                 }
                 __syncthreads();
             }
             if (!overflow) {
-                ncand = misc[2];
+                ncand = P == 1 ? misc[6] : misc[2];
                 break;
             }
             __syncthreads();
@@ -400,7 +466,120 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
 
         // ---- b. candidates: exact top-c ----
         const int ce = min(cq, ncand);
-        if (ce > 0) {
+        if (ce > 0 && P == 1) {
+            // Single pass: the tile holds each occupied slot's sum (kUnset: empty) and
+            // keys_g its id. Keys are (sum desc, id asc). A histogram of the sums over
+            // kSelBins equal-width bins between the smallest and largest sum (a monotone
+            // map of the value) finds the bin holding the ce-th largest: every slot above
+            // it is taken; the boundary bin is resolved exactly by a radix select on its
+            // full keys. cidl first gets slot indices, then ids.
+            uint32_t* hist = reinterpret_cast<uint32_t*>(lds + lo.hist_off);
+            uint64_t* bkeys = reinterpret_cast<uint64_t*>(lds + lo.hist_off);             // kBoundMax keys
+            uint32_t* bslots = reinterpret_cast<uint32_t*>(lds + lo.hist_off) + 2 * kBoundMax;  // after them
+            constexpr int BPT = kSelBins / kSNT;  // bins per thread in the boundary scan
+            if (tid == 0) {
+                misc[12] = 0;  // boundary-bin count (appends)
+                misc[13] = 0;  // boundary-bin appends to scratch (crowded bin)
+            }
+            for (int i = tid; i < kSelBins; i += kSNT) hist[i] = 0;
+            __syncthreads();
+            const float vlo = fkey_inv((uint32_t)misc[7]);
+            const float range = fkey_inv((uint32_t)misc[8]) - vlo;
+            const float scale = (range > 0.0f && range <= 3.0e38f) ? (float)kSelBins / range : 0.0f;
+            auto vbin = [&](uint32_t sb) -> int {  // monotone in the value
+                if (scale == 0.0f) return 0;
+                const float t = (__uint_as_float(sb) - vlo) * scale;
+                return t >= (float)(kSelBins - 1) ? kSelBins - 1 : (int)t;
+            };
+            for (int i = tid; i < T; i += kSNT) {
+                const uint32_t sb = keys[i];
+                if (sb != kUnset) atomicAdd(&hist[vbin(sb)], 1u);
+            }
+            __syncthreads();
+            {  // boundary bin: thread t scans bins [kSelBins - BPT*(t+1), kSelBins - BPT*t) downwards
+                uint32_t c[BPT], tot = 0;
+#pragma unroll
+                for (int j = 0; j < BPT; ++j) {
+                    c[j] = hist[kSelBins - 1 - BPT * tid - j];
+                    tot += c[j];
+                }
+                uint32_t ex = wave_excl_scan(tot, ln);
+                if (ln == kWave - 1) sh.hist[wave] = ex + tot;
+                __syncthreads();
+                for (int w = 0; w < wave; ++w) ex += sh.hist[w];
+                const uint32_t need = (uint32_t)ce;
+                if (ex < need && need <= ex + tot) {
+                    for (int j = 0; j < BPT; ++j) {
+                        if (need <= ex + c[j]) {
+                            misc[9] = kSelBins - 1 - BPT * tid - j;  // boundary bin
+                            misc[10] = (int)ex;                      // slots above it
+                            break;
+                        }
+                        ex += c[j];
+                    }
+                }
+                if (tid == 0) sh.counter = 0;
+            }
+            __syncthreads();
+            const int bbin = misc[9], above = misc[10];
+            // emission: slots above the boundary bin to cidl, boundary-bin slots to bslots
+            for (int i0 = 0; i0 < T; i0 += kEmitU * kSNT) {  // uniform trip count (wave-level append)
+                uint32_t am = 0, bm = 0, ca = 0, cb = 0;
+#pragma unroll
+                for (int u = 0; u < kEmitU; ++u) {
+                    const int i = i0 + u * kSNT + tid;
+                    const uint32_t sb = i < T ? keys[i] : kUnset;
+                    const int bn = sb != kUnset ? vbin(sb) : -1;
+                    am |= (bn > bbin ? 1u : 0u) << u;
+                    bm |= (bn == bbin ? 1u : 0u) << u;
+                }
+                ca = (uint32_t)__popc(am);
+                cb = (uint32_t)__popc(bm);
+                uint32_t pa = wave_append_n<4>(ca, &sh.counter);
+                uint32_t pb = wave_append_n<4>(cb, reinterpret_cast<uint32_t*>(&misc[12]));
+#pragma unroll
+                for (int u = 0; u < kEmitU; ++u) {
+                    const uint32_t i = (uint32_t)(i0 + u * kSNT + tid);
+                    if ((am >> u) & 1u) cidl[pa++] = i;
+                    if ((bm >> u) & 1u) {
+                        if (pb < (uint32_t)kBoundMax) bslots[pb] = i;
+                        ++pb;
+                    }
+                }
+            }
+            __syncthreads();
+            const int nb = misc[12];
+            const int needb = ce - above;
+            if (nb <= kBoundMax) {
+                for (int i = tid; i < nb; i += kSNT) {
+                    const uint32_t sl = bslots[i];
+                    bkeys[i] = ckey(__uint_as_float(keys[sl]), keys_g[sl]);
+                }
+            } else {  // rare: a crowded boundary bin goes through scratch
+                for (int i0 = 0; i0 < T; i0 += kSNT) {
+                    const int i = i0 + tid;
+                    const uint32_t sb = i < T ? keys[i] : kUnset;
+                    const bool tk2 = sb != kUnset && vbin(sb) == bbin;
+                    const uint32_t pos = wave_append(tk2, reinterpret_cast<uint32_t*>(&misc[13]));
+                    if (tk2) cand[pos] = ckey(__uint_as_float(sb), keys_g[i]);
+                }
+            }
+            // ids of the slots above the boundary (one batch of gathers)
+            for (int p2 = tid; p2 < above; p2 += kSNT) cidl[p2] = keys_g[cidl[p2]];
+            __syncthreads();
+            {
+                const bool in_lds = nb <= kBoundMax;
+                auto get = [&](int i) -> uint64_t { return in_lds ? bkeys[i] : cand[i]; };
+                uint64_t prefix;
+                int shift;
+                radix_select<kSNT>(nb, needb, get, sh, prefix, shift);
+                if (tid == 0) sh.counter = (uint32_t)above;
+                __syncthreads();
+                compact_selected<kSNT>(nb, get, prefix, shift, &sh.counter,
+                                       [&](uint32_t pos, uint64_t k) { cidl[pos] = ckey_id(k); });
+            }
+            __syncthreads();
+        } else if (ce > 0) {
             const bool lds_keys = 2 * ncand <= T;  // 8-byte keys fit the (now free) tile
             if (lds_keys) {
                 for (int i = tid; i < ncand; i += kSNT) tile64[i] = cand[i];
@@ -519,7 +698,9 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
     const int DR = dm.D * dm.R;
     const int mp = std::max(1, qb.mp);
     if (mp > kSNT * kBufEntries) return hipErrorInvalidValue;
-    hipError_t err = launch_project_rows(dm, qb.Q, 0, 1, qb.nq, signs, scr.qproj, DR, false, st);
+    hipError_t err = hipSuccess;
+    if (cfg.ev[0] && (err = hipEventRecord(cfg.ev[0], st)) != hipSuccess) return err;
+    err = launch_project_rows(dm, qb.Q, 0, 1, qb.nq, signs, scr.qproj, DR, false, st);
     if (err != hipSuccess) return err;
     {
         const int pgrid = std::min(qb.nq, 4096);
@@ -533,6 +714,7 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
     err = launch_fill_i32(scr.work, 0, 1, st);
     if (err != hipSuccess) return err;
     const int grid = std::max(1, std::min(cfg.grid, qb.nq));
+    if (cfg.ev[1] && (err = hipEventRecord(cfg.ev[1], st)) != hipSuccess) return err;
     switch (search_ept(mp)) {
 #define CEOS_EPT(E)                                                                                            \
     case E:                                                                                                    \
@@ -549,7 +731,9 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
         default:
             return hipErrorInvalidValue;
     }
-    return hipGetLastError();
+    err = hipGetLastError();
+    if (err == hipSuccess && cfg.ev[2]) err = hipEventRecord(cfg.ev[2], st);
+    return err;
 }
 
 }  // namespace ceos

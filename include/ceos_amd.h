@@ -102,6 +102,14 @@ int ceos_project(ceos_index* h, const float* rows, int64_t n, int d, float* out)
 int ceos_set_search_config(ceos_index* h, int wg_per_cu, int table_log2);
 int ceos_get_search_config(const ceos_index* h, int* wg_per_cu, int* table_log2, int* lds_bytes);
 
+/* Kernel timing of coCEOs searches (no reference counterpart; for benchmarks). While
+ * on, each search launch records HIP events on its stream (up to 4096 launches);
+ * turning it on clears earlier records. get_kernel_timing waits for the recorded
+ * launches and returns the summed milliseconds of the front kernels (query
+ * projection + probe selection) and of the fused search kernel, and the count. */
+int ceos_set_kernel_timing(ceos_index* h, int on);
+int ceos_get_kernel_timing(ceos_index* h, double* front_ms, double* fused_ms, int* launches);
+
 /* hipStream_t the index launches on (as void*). */
 void* ceos_get_stream(const ceos_index* h);
 
