@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="repeat the CPU sample for this long")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (counter calibration under a profiler)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -144,6 +146,17 @@ def main():
     wg, tl, lds = index.get_search_config()
     log(f"search config: {wg} workgroups/CU, LDS tile 2^{tl} slots, {lds} B LDS")
 
+    if args.copy_mib:
+        # a plain copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE and
+        # WRITE_SIZE give the scale of the counters against known bytes
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"device copy of {args.copy_mib} MiB done")
+        del src, dst
+
     ids = dI.view(nq, k).cpu().numpy()
     ips = dP.view(nq, k).cpu().numpy()
     exact = exact_topk(torch, dX, dQ, k)
@@ -211,8 +224,16 @@ def main():
     mp = w.m
     bytes_q = 4 * d + 8 * w.s * mp + 4 * d * c_eff + 8 * k
     achieved = bytes_q * nq / (fused_ms / 1e3) / 1e9
+    # measured HBM bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel on
+    # this workload, scaled by a 1 GiB copy (profiles/k_search_fused_<workload>_traffic.json)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"k_search_fused_{w.name}_traffic.json")
+    if os.path.exists(tpath) and n == w.n and nq == w.nq:
+        with open(tpath) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes_per_launch": round(bytes_q * nq),
             "kernel": "k_search_fused", "kernel_ms": round(fused_ms, 4), "bytes_per_query": round(bytes_q),
             "front_kernels_ms": round(front_ms, 4)}
 
