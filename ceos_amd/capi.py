@@ -44,6 +44,13 @@ def lib():
             "ceos_set_search_config": (i32, [vp, i32, i32]),
             "ceos_get_search_config": (i32, [vp, P, P, P]),
             "ceos_get_stream": (vp, [vp]),
+            "ceos_plan_create": (i32, [vp, i32, i32, i32, i32, i32, C.POINTER(C.c_void_p)]),
+            "ceos_stream_plan_create": (i32, [vp, i32, i32, i32, i32, i32, C.POINTER(C.c_void_p)]),
+            "ceos_plan_destroy": (None, [vp]),
+            "ceos_plan_buffers": (i32, [vp, P, P, P, P, P, P]),
+            "ceos_plan_run": (i32, [vp, i32, vp]),
+            "ceos_plan_search": (i32, [vp, P, i32, i32, P, P, P, P, P]),
+            "ceos_plan_config": (i32, [vp, P, P, P]),
             "ceos_set_kernel_timing": (i32, [vp, i32]),
             "ceos_get_kernel_timing": (i32, [vp, P, P, P]),
         }

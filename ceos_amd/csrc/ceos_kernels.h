@@ -78,6 +78,9 @@ struct QueryBatch {
     const int64_t* out_off;  // per-query output offset (null: q * k0)
     int32_t* out_id;
     float* out_ip;
+    // optional: the batch size read on the device (captured plans launch for a maximum
+    // batch nq and take the actual count from here, nq_dev[0] <= nq)
+    const int32_t* nq_dev;
 };
 
 struct QueryConfig {

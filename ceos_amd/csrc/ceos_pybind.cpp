@@ -49,6 +49,63 @@ py::tuple result(int nq, int k, py::array_t<int32_t>& ids, py::array_t<float>& i
     return py::make_tuple(ids, ips);
 }
 
+// A captured batched-search plan (ceos_plan_*); keeps its index alive.
+class PyPlan {
+   public:
+    explicit PyPlan(ceos_plan* p, int d, int k_max) : p_(p), d_(d), k_max_(k_max) {}
+    ~PyPlan() { ceos_plan_destroy(p_); }
+    PyPlan(const PyPlan&) = delete;
+    PyPlan& operator=(const PyPlan&) = delete;
+
+    py::tuple search(FArr Q, IArr k, IArr s, IArr c) {
+        require_2d(Q, "queries");
+        const int nq = (int)Q.shape(0);
+        if (k.size() != nq || s.size() != nq || c.size() != nq)
+            throw py::value_error("k, n_probed_vectors and n_cand arrays must have one entry per query");
+        py::array_t<int32_t> ids({nq, k_max_});
+        py::array_t<float> ips({nq, k_max_});
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_plan_search(p_, Q.data(), nq, (int)Q.shape(1), k.data(), s.data(), c.data(),
+                                  ids.mutable_data(), ips.mutable_data());
+        }
+        check(rc);
+        return py::make_tuple(ids, ips);
+    }
+    void run(int nq, uintptr_t stream) {
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_plan_run(p_, nq, reinterpret_cast<void*>(stream));
+        }
+        check(rc);
+    }
+    py::dict buffers() {
+        float *q, *ip;
+        int32_t *k, *s, *c, *id;
+        check(ceos_plan_buffers(p_, &q, &k, &s, &c, &id, &ip));
+        py::dict out;
+        out["queries"] = reinterpret_cast<uintptr_t>(q);
+        out["k"] = reinterpret_cast<uintptr_t>(k);
+        out["n_probed_vectors"] = reinterpret_cast<uintptr_t>(s);
+        out["n_cand"] = reinterpret_cast<uintptr_t>(c);
+        out["ids"] = reinterpret_cast<uintptr_t>(id);
+        out["ips"] = reinterpret_cast<uintptr_t>(ip);
+        return out;
+    }
+    py::tuple config() const {
+        int wg = 0, tl = 0;
+        double ms = 0.0;
+        check(ceos_plan_config(p_, &wg, &tl, &ms));
+        return py::make_tuple(wg, tl, ms);
+    }
+
+   private:
+    ceos_plan* p_;
+    int d_, k_max_;
+};
+
 class PyCEOs {
    public:
     PyCEOs(int n_points, int n_features) : n_(n_points), d_(n_features) { check(ceos_create(n_points, n_features, &h_)); }
@@ -113,6 +170,12 @@ class PyCEOs {
 
     // Batched form: per-query k, n_probed_vectors and n_cand. Returns (ids, ips, offsets)
     // with query q's results at offsets[q] .. offsets[q+1]-1.
+    PyPlan* make_plan(int nq_max, int k_max, int s_max, int c_max, bool autotune) {
+        ceos_plan* p = nullptr;
+        check(ceos_plan_create(h_, nq_max, k_max, s_max, c_max, autotune ? CEOS_PLAN_AUTOTUNE : 0, &p));
+        return new PyPlan(p, d_, k_max);
+    }
+
     py::tuple search_batched(FArr Q, IArr k, IArr s, IArr c) {
         require_2d(Q, "queries");
         const int nq = (int)Q.shape(0);
@@ -219,6 +282,12 @@ class PyStream {
     explicit PyStream(int n_features) : d_(n_features) { check(ceos_stream_create(n_features, &h_)); }
     ~PyStream() { ceos_stream_destroy(h_); }
 
+    PyPlan* make_plan(int nq_max, int k_max, int s_max, int c_max, bool autotune) {
+        ceos_plan* p = nullptr;
+        check(ceos_stream_plan_create(h_, nq_max, k_max, s_max, c_max, autotune ? CEOS_PLAN_AUTOTUNE : 0, &p));
+        return new PyPlan(p, d_, k_max);
+    }
+
     void setIndexParam(int n_proj, int n_repeats, int top_points, int n_threads, int random_seed) {
         check(ceos_stream_set_index_param(h_, n_proj, n_repeats, top_points, n_threads, random_seed));
     }
@@ -322,6 +391,8 @@ PYBIND11_MODULE(CEOs, m) {
         .def("search_CEOs_Hash",
              [](PyCEOs&, py::object, int, bool) -> py::tuple { not_provided("CEOs-Hash is not provided by this library"); },
              py::arg("queries"), py::arg("n_neighbors"), py::arg("verbose") = false)
+        .def("make_plan", &PyCEOs::make_plan, py::arg("nq_max"), py::arg("k_max"), py::arg("s_max"),
+             py::arg("c_max"), py::arg("autotune") = true, py::keep_alive<0, 1>())
         .def("search_batched", &PyCEOs::search_batched, py::arg("queries"), py::arg("n_neighbors"),
              py::arg("n_probed_vectors"), py::arg("n_cand"))
         .def("get_buckets", &PyCEOs::get_buckets)
@@ -331,6 +402,13 @@ PYBIND11_MODULE(CEOs, m) {
         .def("set_search_config", &PyCEOs::set_search_config, py::arg("wg_per_cu") = 0, py::arg("table_log2") = 0)
         .def("get_search_config", &PyCEOs::get_search_config)
         .def_property_readonly("_handle", &PyCEOs::handle);
+
+    py::class_<PyPlan>(m, "SearchPlan")
+        .def("search", &PyPlan::search, py::arg("queries"), py::arg("n_neighbors"), py::arg("n_probed_vectors"),
+             py::arg("n_cand"))
+        .def("run", &PyPlan::run, py::arg("nq"), py::arg("stream") = 0)
+        .def("buffers", &PyPlan::buffers)
+        .def("config", &PyPlan::config);
 
     py::class_<PyStream>(m, "streamCEOs")
         .def(py::init<int>(), py::arg("n_features"))
@@ -342,6 +420,8 @@ PYBIND11_MODULE(CEOs, m) {
         .def("set_threads", &PyStream::set_threads, py::arg("n_threads"))
         .def("clear", &PyStream::clear)
         .def("build", &PyStream::build, py::arg("dataset"))
+        .def("make_plan", &PyStream::make_plan, py::arg("nq_max"), py::arg("k_max"), py::arg("s_max"),
+             py::arg("c_max"), py::arg("autotune") = true, py::keep_alive<0, 1>())
         .def("update", &PyStream::update, py::arg("new_dataset"), py::arg("n_delPoints") = 0)
         .def("estimate_search", &PyStream::estimate_search, py::arg("queries"), py::arg("n_neighbors"),
              py::arg("verbose") = false)

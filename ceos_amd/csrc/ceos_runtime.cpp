@@ -135,6 +135,35 @@ constexpr int kMaxProbes = 4096;
 constexpr int kMaxK = 4096;
 
 // Device state shared by CEOs and streamCEOs.
+// Device scratch of the coCEOs search for one launch configuration.
+struct ScratchSet {
+    DBuf<uint64_t> cand;
+    DBuf<float> qproj;
+    DBuf<uint2> pairs;
+    DBuf<uint4> stash;
+    DBuf<uint32_t> tkeys;
+    DBuf<int32_t> probes;
+    DBuf<int32_t> work, status;
+
+    ceos::QueryScratch ensure(const ceos::QueryConfig& cfg, int nq, int DR) {
+        cand.ensure((size_t)cfg.grid * cfg.cand_stride);
+        pairs.ensure((size_t)cfg.grid * cfg.cand_stride);
+        stash.ensure((size_t)cfg.grid * ceos::kSearchStashList);
+        tkeys.ensure((size_t)cfg.grid << cfg.table_log2);
+        qproj.ensure((size_t)std::max(nq, 1) * DR);
+        probes.ensure((size_t)std::max(nq, 1) * cfg.s_max);
+        if (!work.p) work.alloc(1);
+        if (!status.p) {
+            status.alloc(1);
+            HIP_CHECK(hipMemset(status.p, 0, 4));
+        }
+        return ceos::QueryScratch{qproj.p, probes.p, cand.p, pairs.p, stash.p, tkeys.p, work.p, status.p};
+    }
+    void release() {
+        cand.release(); qproj.release(); pairs.release(); stash.release(); tkeys.release(); probes.release();
+    }
+};
+
 struct Core {
     int d = 0;
     int D = 256, R = 1, m = 100, F = 0, L = 0;
@@ -158,13 +187,7 @@ struct Core {
     DBuf<uint32_t> dead;  // stream random-delete bitmap over ring slots
 
     // query scratch
-    DBuf<uint64_t> cand;
-    DBuf<float> qproj;
-    DBuf<uint2> pairs;
-    DBuf<uint4> stash;
-    DBuf<uint32_t> tkeys;
-    DBuf<int32_t> probes;
-    DBuf<int32_t> work, status;
+    ScratchSet scr;
     int wg_per_cu = 0, table_log2 = 0;
     int last_lds = 0, last_tlog2 = 0, last_wg = 0;
     // optional per-launch kernel timing (HIP events on the launch stream)
@@ -225,7 +248,7 @@ struct Core {
 
     void clear() {
         bkt.release(); bsize.release(); X.release(); signs.release(); dead.release();
-        cand.release(); qproj.release(); probes.release(); pairs.release(); stash.release(); tkeys.release(); work.release(); status.release(); estP.release();
+        scr.release(); estP.release();
         bits.clear();
         cap = head = 0; base = live = 0; est_n = 0;
         built = false;
@@ -324,11 +347,17 @@ struct Core {
     // the largest tile that fits the per-workgroup LDS share (wg_per_cu workgroups
     // of 512 threads per CU; default 1, i.e. a 32K-slot tile of 4-byte ids).
     ceos::QueryConfig search_config(int s_max, int c_max, int k_max) {
-        const int wg = wg_per_cu > 0 ? wg_per_cu : 1;
+        ceos::QueryConfig cfg = search_config_with(s_max, c_max, k_max, wg_per_cu, table_log2);
+        arm_timing(cfg);
+        return cfg;
+    }
+    // The same for an explicit launch shape (0 = automatic), without kernel timing.
+    ceos::QueryConfig search_config_with(int s_max, int c_max, int k_max, int wg_req, int tl_req) {
+        const int wg = wg_req > 0 ? wg_req : 1;
         const int budget = std::min(kMaxLdsBytes, (160 * 1024) / wg - 2048);
         int t = 0;
-        if (table_log2 > 0) {
-            t = table_log2;
+        if (tl_req > 0) {
+            t = tl_req;
         } else {
             for (int cand_t = 15; cand_t >= 8; --cand_t)
                 if (ceos::search_lds_bytes(d, cand_t, s_max, c_max, k_max) <= budget) { t = cand_t; break; }
@@ -350,29 +379,18 @@ struct Core {
         cfg.s_max = s_max; cfg.c_max = c_max; cfg.k_max = k_max;
         cfg.cand_stride = (int64_t)s_max * std::max(1, std::min(mp, m));
         last_lds = lds; last_tlog2 = t; last_wg = std::min(wg, fit);
-        arm_timing(cfg);
         return cfg;
     }
 
-    ceos::QueryScratch scratch(const ceos::QueryConfig& cfg, int nq) {
-        cand.ensure((size_t)cfg.grid * cfg.cand_stride);
-        pairs.ensure((size_t)cfg.grid * cfg.cand_stride);
-        stash.ensure((size_t)cfg.grid * ceos::kSearchStashList);
-        tkeys.ensure((size_t)cfg.grid << cfg.table_log2);
-        qproj.ensure((size_t)std::max(nq, 1) * DR());
-        probes.ensure((size_t)std::max(nq, 1) * cfg.s_max);
-        if (!work.p) work.alloc(1);
-        if (!status.p) { status.alloc(1); HIP_CHECK(hipMemset(status.p, 0, 4)); }
-        return ceos::QueryScratch{qproj.p, probes.p, cand.p, pairs.p, stash.p, tkeys.p, work.p, status.p};
-    }
+    ceos::QueryScratch scratch(const ceos::QueryConfig& cfg, int nq) { return scr.ensure(cfg, nq, DR()); }
 
     ceos::RowRing rows() const { return ceos::RowRing{X.p, cap, head, base, live}; }
 
     void check_status() {
         int32_t flag = 0;
-        HIP_CHECK(hipMemcpy(&flag, status.p, 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(&flag, scr.status.p, 4, hipMemcpyDeviceToHost));
         if (flag) {
-            HIP_CHECK(hipMemset(status.p, 0, 4));
+            HIP_CHECK(hipMemset(scr.status.p, 0, 4));
             throw DevError("LDS score tile overflowed; rerun with a larger table (ceos_set_search_config)");
         }
     }
@@ -453,6 +471,160 @@ struct ceos_index {
 struct ceos_stream {
     Core core;
 };
+
+// A batched search captured in a hipGraph over plan-owned buffers (see ceos_plan_create).
+struct ceos_plan {
+    Core* core = nullptr;
+    bool stream_index = false;  // streamCEOs: n_probed_vectors is bounded by n_proj
+    int nq_max = 0, k_max = 0, s_max = 0, c_max = 0, flags = 0;
+    DBuf<float> Q, ips;
+    DBuf<int32_t> k, s, c, ids, nq_dev;
+    ScratchSet scr;
+    ceos::QueryConfig cfg{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr;
+    // everything the captured launches hold by value
+    struct Snap {
+        const void *signs, *bkt, *bsize, *X;
+        int64_t cap, head;
+        uint32_t base, live;
+        int mp, m, D, R, L, d;
+        bool operator==(const Snap& o) const {
+            return signs == o.signs && bkt == o.bkt && bsize == o.bsize && X == o.X && cap == o.cap &&
+                   head == o.head && base == o.base && live == o.live && mp == o.mp && m == o.m && D == o.D &&
+                   R == o.R && L == o.L && d == o.d;
+        }
+    } snap{};
+    bool tuned = false;
+    double tuned_ms = 0.0;
+    int wg = 0, tl = 0;
+
+    ~ceos_plan() {
+        drop();
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+    void drop() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    Snap current() const {
+        const Core& x = *core;
+        return Snap{x.signs.p, x.bkt.p, x.bsize.p, x.X.p, x.cap, x.head, x.base, x.live, x.mp, x.m, x.D, x.R, x.L, x.d};
+    }
+    void capture(int wg_, int tl_) {
+        drop();
+        Core& x = *core;
+        cfg = x.search_config_with(s_max, c_max, k_max, wg_, tl_);
+        const ceos::QueryScratch sc = scr.ensure(cfg, nq_max, x.DR());
+        // k0 = k_max: rows at q * k_max, zero padded to k_max
+        ceos::QueryBatch qb{Q.p, nq_max, k.p, s.p, c.p, k_max, 0, 0, x.mp, nullptr, ids.p, ips.p, nq_dev.p};
+        if (!cap) HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        const hipError_t e1 =
+            ceos::launch_coceos_search(x.dims(), x.signs.p, x.bkt.p, x.bsize.p, x.rows(), qb, cfg, sc, cap);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(cap, &g);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            throw DevError(std::string("search plan capture failed: ") +
+                           hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        }
+        graph = g;
+        HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        snap = current();
+        wg = wg_;
+        tl = tl_;
+    }
+    void launch(int nq, hipStream_t st) {
+        HIP_CHECK(ceos::launch_fill_i32(nq_dev.p, nq, 1, st));  // a kernel, outside the graph
+        HIP_CHECK(hipGraphLaunch(exec, st));
+    }
+    // Times each candidate launch shape once on this batch and keeps the fastest. Every
+    // candidate computes the same results, so the batch's outputs are valid afterwards.
+    void tune(int nq, hipStream_t st) {
+        Core& x = *core;
+        const int t1 = x.search_config_with(s_max, c_max, k_max, 1, 0).table_log2;
+        const int t2 = x.search_config_with(s_max, c_max, k_max, 2, 0).table_log2;
+        std::vector<std::pair<int, int>> cands = {{1, 0}, {2, 0}};
+        if (t1 - 1 >= 8) cands.push_back({1, t1 - 1});
+        if (t2 - 1 >= 8) cands.push_back({2, t2 - 1});
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        double best = 1e30;
+        std::pair<int, int> pick = cands[0];
+        try {
+            for (const auto& cd : cands) {
+                capture(cd.first, cd.second);
+                launch(nq, st);  // warm
+                HIP_CHECK(hipEventRecord(e0, st));
+                launch(nq, st);
+                HIP_CHECK(hipEventRecord(e1, st));
+                HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    pick = cd;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        capture(pick.first, pick.second);
+        tuned = true;
+        tuned_ms = best;
+    }
+    void validate() const {
+        const Core& x = *core;
+        x.validate_query(s_max, x.mp, c_max, k_max, stream_index ? x.D : x.DR(),
+                         stream_index ? "number of projections" : "n_proj * n_repeats");
+    }
+    void run(int nq, hipStream_t st) {
+        if (nq < 0 || nq > nq_max) throw ArgError("batch of " + std::to_string(nq) + " exceeds the plan's nq_max");
+        validate();
+        if (nq == 0) return;
+        if (!exec || !(snap == current())) capture(wg, tl);
+        if ((flags & CEOS_PLAN_AUTOTUNE) && !tuned)
+            tune(nq, st);
+        else
+            launch(nq, st);
+    }
+};
+
+namespace {
+int make_plan(Core& x, bool stream_index, int nq_max, int k_max, int s_max, int c_max, int flags, ceos_plan** out) {
+    return guarded([&] {
+        if (!out) throw ArgError("null output handle");
+        if (nq_max <= 0 || k_max <= 0 || s_max <= 0 || c_max <= 0)
+            throw ArgError("nq_max, k_max, s_max and c_max must be positive");
+        std::lock_guard<std::mutex> g(x.mu);
+        auto* p = new ceos_plan();
+        try {
+            p->core = &x;
+            p->stream_index = stream_index;
+            p->nq_max = nq_max; p->k_max = k_max; p->s_max = s_max; p->c_max = c_max; p->flags = flags;
+            if (x.built) p->validate();
+            p->Q.alloc((size_t)nq_max * x.d);
+            p->k.alloc(nq_max); p->s.alloc(nq_max); p->c.alloc(nq_max);
+            p->ids.alloc((size_t)nq_max * k_max);
+            p->ips.alloc((size_t)nq_max * k_max);
+            p->nq_dev.alloc(1);
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+}  // namespace
 
 extern "C" {
 
@@ -684,15 +856,15 @@ int ceos_search_est(ceos_index* h, const float* Q, int nq, int d, int k, int32_t
         const int64_t stride = c.est_n + cc;
         int grid = device_cus() * 2;
         grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t(2) << 30) / (stride * 8)));
-        c.cand.ensure((size_t)grid * stride);
-        if (!c.work.p) c.work.alloc(1);
+        c.scr.cand.ensure((size_t)grid * stride);
+        if (!c.scr.work.p) c.scr.work.alloc(1);
         DBuf<float> dQ((size_t)nq * d);
         DBuf<int32_t> dI((size_t)nq * k);
         DBuf<float> dP((size_t)nq * k);
         HIP_CHECK(hipMemcpyAsync(dQ.p, Q, (size_t)nq * d * 4, hipMemcpyHostToDevice, c.st()));
         ceos::QueryBatch qb{dQ.p, nq, nullptr, nullptr, nullptr, k, c.s, cc, 0, nullptr, dI.p, dP.p};
-        HIP_CHECK(ceos::launch_est_search(c.dims(), c.signs.p, c.estP.p, c.est_n, c.X.p, qb, grid, c.cand.p, stride,
-                                          c.work.p, c.st()));
+        HIP_CHECK(ceos::launch_est_search(c.dims(), c.signs.p, c.estP.p, c.est_n, c.X.p, qb, grid, c.scr.cand.p, stride,
+                                          c.scr.work.p, c.st()));
         HIP_CHECK(hipMemcpyAsync(ids, dI.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c.st()));
         HIP_CHECK(hipMemcpyAsync(ips, dP.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, c.st()));
         HIP_CHECK(hipStreamSynchronize(c.st()));
@@ -993,6 +1165,92 @@ int ceos_stream_get_buckets(const ceos_stream* h, int32_t* cid, float* cval, int
         auto& c = const_cast<Core&>(h->core);
         std::lock_guard<std::mutex> g(c.mu);
         c.get_buckets(cid, cval, fid, fval, csz, fsz);
+    });
+}
+
+// ------------------------------- search plans ------------------------------------
+
+int ceos_plan_create(ceos_index* h, int nq_max, int k_max, int s_max, int c_max, int flags, ceos_plan** out) {
+    if (!h) {
+        g_err = "null handle";
+        return CEOS_EINVAL;
+    }
+    return make_plan(h->core, false, nq_max, k_max, s_max, c_max, flags, out);
+}
+
+int ceos_stream_plan_create(ceos_stream* h, int nq_max, int k_max, int s_max, int c_max, int flags,
+                            ceos_plan** out) {
+    if (!h) {
+        g_err = "null handle";
+        return CEOS_EINVAL;
+    }
+    return make_plan(h->core, true, nq_max, k_max, s_max, c_max, flags, out);
+}
+
+void ceos_plan_destroy(ceos_plan* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(p->core->mu);
+    delete p;
+}
+
+int ceos_plan_buffers(ceos_plan* p, float** dQ, int32_t** dk, int32_t** ds, int32_t** dc, int32_t** d_ids,
+                      float** d_ips) {
+    return guarded([&] {
+        if (!p) throw ArgError("null plan");
+        if (dQ) *dQ = p->Q.p;
+        if (dk) *dk = p->k.p;
+        if (ds) *ds = p->s.p;
+        if (dc) *dc = p->c.p;
+        if (d_ids) *d_ids = p->ids.p;
+        if (d_ips) *d_ips = p->ips.p;
+    });
+}
+
+int ceos_plan_run(ceos_plan* p, int nq, void* stream) {
+    return guarded([&] {
+        if (!p) throw ArgError("null plan");
+        std::lock_guard<std::mutex> g(p->core->mu);
+        p->run(nq, p->core->stream(stream));
+    });
+}
+
+int ceos_plan_search(ceos_plan* p, const float* Q, int nq, int d, const int32_t* k, const int32_t* s,
+                     const int32_t* c, int32_t* ids, float* ips) {
+    return guarded([&] {
+        if (!p) throw ArgError("null plan");
+        std::lock_guard<std::mutex> g(p->core->mu);
+        Core& x = *p->core;
+        check_matrix(Q, nq, d, x.d);
+        if (nq > p->nq_max) throw ArgError("batch of " + std::to_string(nq) + " exceeds the plan's nq_max");
+        if (nq <= 0) return;
+        if (!k || !s || !c || !ids || !ips) throw ArgError("null array");
+        for (int q = 0; q < nq; ++q)
+            if (k[q] < 0 || k[q] > p->k_max || s[q] <= 0 || s[q] > p->s_max || c[q] <= 0 || c[q] > p->c_max)
+                throw ArgError("query " + std::to_string(q) + ": k, n_probed_vectors or n_cand outside the plan's bounds");
+        hipStream_t st = x.st();
+        HIP_CHECK(hipMemcpyAsync(p->Q.p, Q, (size_t)nq * d * 4, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(p->k.p, k, (size_t)nq * 4, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(p->s.p, s, (size_t)nq * 4, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(p->c.p, c, (size_t)nq * 4, hipMemcpyHostToDevice, st));
+        p->run(nq, st);
+        HIP_CHECK(hipMemcpyAsync(ids, p->ids.p, (size_t)nq * p->k_max * 4, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(ips, p->ips.p, (size_t)nq * p->k_max * 4, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        int32_t flag = 0;
+        HIP_CHECK(hipMemcpy(&flag, p->scr.status.p, 4, hipMemcpyDeviceToHost));
+        if (flag) {
+            HIP_CHECK(hipMemset(p->scr.status.p, 0, 4));
+            throw DevError("LDS score tile overflowed");
+        }
+    });
+}
+
+int ceos_plan_config(const ceos_plan* p, int* wg_per_cu, int* table_log2, double* tuned_ms) {
+    return guarded([&] {
+        if (!p) throw ArgError("null plan");
+        if (wg_per_cu) *wg_per_cu = p->cfg.grid > 0 ? p->cfg.grid / std::max(1, device_cus()) : 0;
+        if (table_log2) *table_log2 = p->cfg.table_log2;
+        if (tuned_ms) *tuned_ms = p->tuned ? p->tuned_ms : 0.0;
     });
 }
 

@@ -95,10 +95,12 @@ int search_lds_bytes(int d, int table_log2, int s_max, int c_max, int k_max) {
 
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(kPNT) void k_probe_select(int DR, const float* __restrict__ qproj, int nq,
+                                                       const int32_t* __restrict__ nq_dev,
                                                        const int32_t* __restrict__ s_arr, int s0, int s_stride,
                                                        int32_t* __restrict__ probes) {
     __shared__ SelectShared sh;
     extern __shared__ uint64_t pkey[];
+    if (nq_dev) nq = min(nq, nq_dev[0]);
     for (int q = blockIdx.x; q < nq; q += gridDim.x) {
         const int sq = min(max(s_arr ? s_arr[q] : s0, 0), min(s_stride, DR));
         if (sq == 0) continue;  // uniform
@@ -158,13 +160,14 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
     uint4* stash_list = scr.stash + (int64_t)blockIdx.x * kStashList;
     const int tid = threadIdx.x, wave = tid / kWave, ln = tid % kWave;
     float frac = 0.5f;  // running estimate of distinct ids / entries for this workgroup
+    const int nq_run = qb.nq_dev ? min(qb.nq, qb.nq_dev[0]) : qb.nq;
 
     for (;;) {
         if (tid == 0) misc[0] = atomicAdd(scr.work, 1);
         __syncthreads();
         const int q = misc[0];
         __syncthreads();
-        if (q >= qb.nq) break;
+        if (q >= nq_run) break;
         // per-query parameters, clamped to the bounds the launch was sized for
         const int kq = min(max(qb.k ? qb.k[q] : qb.k0, 0), cfg.k_max);
         const int sq = min(max(qb.s ? qb.s[q] : qb.s0, 0), cfg.s_max);
@@ -677,7 +680,8 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             __syncthreads();
             bitonic_sort_desc<kSNT>(tk, N);
         }
-        for (int t = tid; t < kq; t += kSNT) {
+        const int kw = qb.out_off ? kq : qb.k0;  // fixed-stride rows are zero padded to the stride
+        for (int t = tid; t < kw; t += kSNT) {
             qb.out_id[off + t] = t < kk ? (int32_t)ckey_id(tk[t]) : 0;
             qb.out_ip[off + t] = t < kk ? ckey_val(tk[t]) : 0.0f;
         }
@@ -706,7 +710,8 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
         const int pgrid = std::min(qb.nq, 4096);
         const size_t plds = (size_t)next_pow2(cfg.s_max) * 8;
         allow_lds_s(k_probe_select, plds);
-        hipLaunchKernelGGL(k_probe_select, dim3(pgrid), dim3(kPNT), plds, st, DR, scr.qproj, qb.nq, qb.s, qb.s0,
+        hipLaunchKernelGGL(k_probe_select, dim3(pgrid), dim3(kPNT), plds, st, DR, scr.qproj, qb.nq, qb.nq_dev,
+                           qb.s, qb.s0,
                            cfg.s_max, scr.probes);
         err = hipGetLastError();
         if (err != hipSuccess) return err;

@@ -113,6 +113,40 @@ int ceos_get_kernel_timing(ceos_index* h, double* front_ms, double* fused_ms, in
 /* hipStream_t the index launches on (as void*). */
 void* ceos_get_stream(const ceos_index* h);
 
+/* ------------------------- captured search plans -------------------------
+ * No reference counterpart: the batched form of search_coCEOs_Est / estimate_search
+ * for callers that search repeatedly with batches of up to nq_max queries. A plan owns
+ * device buffers for the queries (nq_max x d), the per-query k / n_probed_vectors /
+ * n_cand (nq_max each, bounded by k_max / s_max / c_max) and the results (query q's
+ * row at q*k_max, k[q] entries then zero padded to k_max), plus its own scratch. Its
+ * launches (query projection, probe selection, fused search) are captured once in a
+ * hipGraph and replayed by ceos_plan_run with the batch size read on the device.
+ * With CEOS_PLAN_AUTOTUNE the first run times the candidate launch shapes
+ * (workgroups per CU, score-tile size) on that batch and keeps the fastest; results
+ * do not depend on the shape. A plan re-captures itself when the index changes (build,
+ * update, delete, n_probed_points). Runs of one plan are ordered on the given stream;
+ * the plan must not be destroyed while a run is in flight. */
+typedef struct ceos_plan ceos_plan;
+#define CEOS_PLAN_AUTOTUNE 1
+
+int ceos_plan_create(ceos_index* h, int nq_max, int k_max, int s_max, int c_max, int flags, ceos_plan** out);
+int ceos_stream_plan_create(ceos_stream* h, int nq_max, int k_max, int s_max, int c_max, int flags,
+                            ceos_plan** out);
+void ceos_plan_destroy(ceos_plan* p);
+/* Device buffers of the plan: Q (nq_max x d FP32), k, s, c (nq_max int32 each), ids and
+ * ips (nq_max x k_max). Any output pointer may be NULL. */
+int ceos_plan_buffers(ceos_plan* p, float** dQ, int32_t** dk, int32_t** ds, int32_t** dc, int32_t** d_ids,
+                      float** d_ips);
+/* Replays the plan for the first nq queries of its buffers on `stream` (NULL: the
+ * index's stream); does not synchronise. */
+int ceos_plan_run(ceos_plan* p, int nq, void* stream);
+/* Host convenience: copies Q, k, s, c in, runs, copies the results out (ids/ips as
+ * nq x k_max) and synchronises. */
+int ceos_plan_search(ceos_plan* p, const float* Q, int nq, int d, const int32_t* k, const int32_t* s,
+                     const int32_t* c, int32_t* ids, float* ips);
+/* Launch shape in use (after tuning) and the tuned kernel time in ms (0 if untuned). */
+int ceos_plan_config(const ceos_plan* p, int* wg_per_cu, int* table_log2, double* tuned_ms);
+
 /* --------------------------- class streamCEOs --------------------------- */
 
 /* streamCEOs(n_features) — python_wrapper.cpp:53, streamCEOs.h:47-51 */
