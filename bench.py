@@ -1,17 +1,21 @@
 #!/usr/bin/env python3
 """coCEOs search benchmark on one MI355X.
 
-A step is one batched search of the workload's whole query set (all queries in one
-fused persistent launch), with the points, buckets and queries already resident in
-HBM. Reported beside it, from the same run:
+A step is one batched search of the workload's whole query set, replayed from a
+captured plan (hipGraph of query projection + probe selection + the fused persistent
+search, launch shape autotuned on the first run), with the points, buckets, queries
+and per-query (k, n_probed_vectors, n_cand) already resident in HBM. Reported beside
+it, from the same run:
   * plain per-query GPU path: the same index searched one query per call,
     synchronising after each (the drop-in search used query by query);
   * build throughput (Mpoints/s) of the device build from HBM-resident points;
-  * recall@10 against exact MIPS (FP32 matmul on the device);
-  * roofline of the fused search kernel (algorithmic HBM bytes per launch / its
-    average duration from HIP events on its stream);
+  * recall@k against exact MIPS (FP32 matmul on the device);
+  * roofline of the fused search kernel: algorithmic HBM bytes per launch over its
+    average duration, from HIP events the library records around it on its stream,
+    and the measured HBM traffic of that kernel (rocprofv3 counters, from the
+    committed profile of the same workload);
   * the CPU baseline: the oracle (a C++/OpenMP restatement of the reference's
-    search_coCEOs_Est) timed on the host on a bounded query sample of the same index.
+    search_coCEOs_Est) timed on the host on the same index and queries.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -58,6 +62,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="repeat the CPU sample for this long")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (counter calibration under a profiler)")
     args = ap.parse_args()
@@ -70,16 +75,19 @@ def main():
     import torch
 
     from ceos_amd import CEOs, capi
-    from ceos_amd.workloads import WORKLOADS, make_points, make_queries, recall
+    from ceos_amd.workloads import WORKLOADS, make_points, make_queries, make_query_params
 
     w = WORKLOADS[args.workload]
     n = args.n or w.n
     nq = args.nq or w.nq
-    d, k = w.d, w.k
+    d = w.d
     dev = torch.device("cuda:0")
     torch.cuda.init()
 
-    log(f"workload {w.name}: n={n} d={d} nq={nq} D={w.D} R={w.R} m={w.m} s={w.s} c={w.c} k={k}")
+    ks, ss, cs = make_query_params(w, nq)
+    k_max, s_max, c_max = int(ks.max()), int(ss.max()), int(cs.max())
+    log(f"workload {w.name}: n={n} d={d} nq={nq} D={w.D} R={w.R} m={w.m} "
+        f"k in {sorted(set(ks.tolist()))[:3]} s in [{ss.min()},{s_max}] c in [{cs.min()},{c_max}]")
     t = time.time()
     X = make_points(w, n)
     Q = make_queries(w, nq)
@@ -109,11 +117,20 @@ def main():
     index.n_probed_points = w.m
     index.n_cand = w.c
 
-    dI = torch.empty(nq * k, dtype=torch.int32, device=dev)
-    dP = torch.empty(nq * k, dtype=torch.float32, device=dev)
+    # ---- batched: a captured plan. The first search copies the queries and per-query
+    # parameters into the plan's device buffers and autotunes the launch shape; the timed
+    # steps replay the plan on those resident buffers.
+    plan = index.make_plan(nq, k_max, s_max, c_max, autotune=not args.no_autotune)
+    t0 = time.perf_counter()
+    ids, _ = plan.search(Q, ks, ss, cs)  # rows of k_max: k[q] results then zeros
+    log(f"plan first run (with autotune) {time.perf_counter() - t0:.2f}s; "
+        f"(wg/CU, tile log2, tuned ms) = {plan.config()}")
+    dk = torch.from_numpy(ks).to(dev)
+    ds = torch.from_numpy(ss).to(dev)
+    dc = torch.from_numpy(cs).to(dev)
 
     def step():
-        capi.check(L.ceos_search_coceos_device(h, dQ.data_ptr(), nq, d, k, dI.data_ptr(), dP.data_ptr(), stream))
+        plan.run(nq, stream)
 
     for _ in range(args.warmup):
         step()
@@ -129,12 +146,29 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = ev0.elapsed_time(ev1) / args.steps
     qps = nq / (ms_step / 1e3)
-    log(f"batched: {ms_step:.3f} ms/step (events), wall {wall / args.steps * 1e3:.3f} ms/step, {qps:,.0f} QPS")
-    # the fused search kernel alone: HIP events recorded by the library on the launch
-    # stream around each kernel, over a second timed run of the same steps
+    log(f"batched (graph): {ms_step:.3f} ms/step (events), wall {wall / args.steps * 1e3:.3f} ms/step, "
+        f"{qps:,.0f} QPS")
+    wg, tl, _ = plan.config()
+
+    # ---- the fused kernel alone, same launch shape, direct launches with kernel events
+    index.set_search_config(wg, tl)
+    off = np.zeros(nq, np.int64)
+    off[1:] = np.cumsum(ks)[:-1]
+    doff = torch.from_numpy(off).to(dev)
+    tot = int(ks.sum())
+    dI2 = torch.empty(max(1, tot), dtype=torch.int32, device=dev)
+    dP2 = torch.empty(max(1, tot), dtype=torch.float32, device=dev)
+
+    def direct():
+        capi.check(L.ceos_search_batched_device(h, dQ.data_ptr(), nq, d, dk.data_ptr(), ds.data_ptr(), dc.data_ptr(),
+                                                doff.data_ptr(), k_max, s_max, c_max, dI2.data_ptr(),
+                                                dP2.data_ptr(), stream))
+
+    direct()
+    torch.cuda.synchronize()
     capi.check(L.ceos_set_kernel_timing(h, 1))
     for _ in range(args.steps):
-        step()
+        direct()
     front = ctypes.c_double()
     fused = ctypes.c_double()
     nl = ctypes.c_int()
@@ -143,8 +177,7 @@ def main():
     fused_ms = fused.value / max(1, nl.value)
     front_ms = front.value / max(1, nl.value)
     log(f"kernel events: fused search {fused_ms:.3f} ms/launch, projection+probe select {front_ms:.3f} ms/launch")
-    wg, tl, lds = index.get_search_config()
-    log(f"search config: {wg} workgroups/CU, LDS tile 2^{tl} slots, {lds} B LDS")
+    index.set_search_config(0, 0)
 
     if args.copy_mib:
         # a plain copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE and
@@ -157,38 +190,45 @@ def main():
         log(f"device copy of {args.copy_mib} MiB done")
         del src, dst
 
-    ids = dI.view(nq, k).cpu().numpy()
-    ips = dP.view(nq, k).cpu().numpy()
-    exact = exact_topk(torch, dX, dQ, k)
-    rec = recall(exact, ids)
-    log(f"recall@{k} = {rec:.4f}")
+    # ---- recall against exact MIPS at each query's own k (and @10 where k >= 10)
+    exact = exact_topk(torch, dX, dQ, k_max)
+    rec_k = float(np.mean([len(np.intersect1d(exact[q, :ks[q]], ids[q, :ks[q]])) / ks[q]
+                           for q in range(nq) if ks[q] > 0]))
+    sel10 = np.nonzero(ks >= 10)[0]
+    rec10 = (float(np.mean([len(np.intersect1d(exact[q, :10], ids[q, :10])) / 10.0 for q in sel10]))
+             if len(sel10) else None)
+    log(f"recall@k (own k) = {rec_k:.4f}; recall@10 over queries with k>=10 = {rec10}")
 
-    # plain per-query path: one call per query, synchronised
+    # ---- plain per-query path: one call per query (its own parameters), synchronised
     P = min(args.plain_queries, nq)
     plain_qps, lat = None, np.zeros(1)
     if P > 0:
-        dI1 = torch.empty(k, dtype=torch.int32, device=dev)
-        dP1 = torch.empty(k, dtype=torch.float32, device=dev)
+        dI1 = torch.empty(k_max, dtype=torch.int32, device=dev)
+        dP1 = torch.empty(k_max, dtype=torch.float32, device=dev)
+
+        def one(i):
+            capi.check(L.ceos_set_query_param(h, int(ss[i]), w.m, int(cs[i])))
+            capi.check(L.ceos_search_coceos_device(h, dQ[i].data_ptr(), 1, d, int(ks[i]), dI1.data_ptr(),
+                                                   dP1.data_ptr(), stream))
+            torch.cuda.synchronize()
+
         for i in range(3):
-            capi.check(L.ceos_search_coceos_device(h, dQ[i].data_ptr(), 1, d, k, dI1.data_ptr(), dP1.data_ptr(),
-                                                   stream))
-        torch.cuda.synchronize()
+            one(i)
         lat = []
         t0 = time.perf_counter()
         for i in range(P):
             a = time.perf_counter()
-            capi.check(L.ceos_search_coceos_device(h, dQ[i].data_ptr(), 1, d, k, dI1.data_ptr(), dP1.data_ptr(),
-                                                   stream))
-            torch.cuda.synchronize()
+            one(i)
             lat.append(time.perf_counter() - a)
         plain_qps = P / (time.perf_counter() - t0)
         lat = np.array(lat) * 1e3
         log(f"plain per-query: {plain_qps:,.0f} QPS, p50 {np.percentile(lat, 50):.3f} ms "
             f"p99 {np.percentile(lat, 99):.3f} ms")
+        capi.check(L.ceos_set_query_param(h, w.s, w.m, w.c))
 
-    # CPU baseline (oracle restatement of the reference search) on a bounded sample
+    # ---- CPU baseline (oracle restatement of the reference search), same index
     cpu = None
-    c_eff = float(w.c)
+    c_eff = cs.astype(np.float64)
     if not args.no_cpu_baseline:
         from oracle import oracle as O
 
@@ -199,18 +239,19 @@ def main():
         o.load_index(X, cid, cv, fid, fv)
         o.n_probed_vectors, o.n_probed_points, o.n_cand = w.s, w.m, w.c
         S = min(args.cpu_queries or nq, nq)
-        o.search_coCEOs_Est(Q[:64], k)  # warm caches / thread pool
-        # repeat the sample until about --cpu-seconds of host work is timed
+        o.search_coCEOs_Est(Q[:64], 0, ks=ks[:64], ss=ss[:64], cs=cs[:64])  # warm caches / thread pool
         t0 = time.perf_counter()
-        oi, op, distinct, _ = o.search_coCEOs_Est(Q[:S], k, stats=True)
+        oi, op, distinct, _ = o.search_coCEOs_Est(Q[:S], 0, ks=ks[:S], ss=ss[:S], cs=cs[:S], stats=True)
         reps = 1
         while time.perf_counter() - t0 < args.cpu_seconds:
-            o.search_coCEOs_Est(Q[:S], k)
+            o.search_coCEOs_Est(Q[:S], 0, ks=ks[:S], ss=ss[:S], cs=cs[:S])
             reps += 1
         cpu_s = time.perf_counter() - t0
         cpu_qps = S * reps / cpu_s
-        match = float(np.mean(np.all(oi == ids[:S], axis=1)))
-        c_eff = float(np.mean(np.minimum(w.c, distinct)))
+        ooff = np.zeros(S + 1, np.int64)
+        ooff[1:] = np.cumsum(ks[:S])
+        match = float(np.mean([np.array_equal(oi[ooff[q]:ooff[q + 1]], ids[q, :ks[q]]) for q in range(S)]))
+        c_eff[:S] = np.minimum(cs[:S], distinct)
         log(f"cpu oracle: {cpu_qps:,.0f} QPS on {S} queries x {reps}, {threads} threads; top-k match {match:.4f}; "
             f"mean distinct candidates {distinct.mean():.0f}")
         cpu = {"value": round(cpu_qps, 1), "unit": "queries/s", "cores": threads, "kind": "port",
@@ -218,12 +259,12 @@ def main():
                          f"build, verified identical to the oracle's build in tests); {cpu_s:.1f} s",
                "topk_match_vs_gpu": match}
 
-    # roofline of the fused search kernel: algorithmic bytes per query are the query
-    # row, the probed bucket entries (8 B each), the candidates' rows and the top-k
-    # output; one launch covers all nq queries
+    # ---- roofline of the fused search kernel. Algorithmic bytes per query: the query row,
+    # the probed bucket entries (8 B each), the candidates' rows and the top-k output;
+    # one launch covers all nq queries.
     mp = w.m
-    bytes_q = 4 * d + 8 * w.s * mp + 4 * d * c_eff + 8 * k
-    achieved = bytes_q * nq / (fused_ms / 1e3) / 1e9
+    bytes_launch = float(np.sum(4 * d + 8 * ss.astype(np.float64) * mp + 4 * d * c_eff + 8 * ks))
+    achieved = bytes_launch / (fused_ms / 1e3) / 1e9
     # measured HBM bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel on
     # this workload, scaled by a 1 GiB copy (profiles/k_search_fused_<workload>_traffic.json)
     traffic = None
@@ -233,10 +274,11 @@ def main():
             traffic = json.load(f).get("traffic_bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "algorithmic_bytes_per_launch": round(bytes_q * nq),
-            "kernel": "k_search_fused", "kernel_ms": round(fused_ms, 4), "bytes_per_query": round(bytes_q),
-            "front_kernels_ms": round(front_ms, 4)}
+            "algorithmic_bytes_per_launch": round(bytes_launch),
+            "kernel": "k_search_fused", "kernel_ms": round(fused_ms, 4),
+            "bytes_per_query": round(bytes_launch / nq), "front_kernels_ms": round(front_ms, 4)}
 
+    uneven = bool(w.uneven)
     out = {
         "metric": METRIC,
         "value": round(qps, 1),
@@ -249,11 +291,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (numpy PCG64 seeds 0/1; index sign seed 42)",
+        "data": "synthetic (numpy PCG64 seeds 0/1/2; index sign seed 42)",
         "config": {"workload": w.name, "n": n, "d": d, "queries": nq, "n_proj": w.D, "n_repeats": w.R,
-                   "top_m": w.m, "n_probed_vectors": w.s, "n_probed_points": mp, "n_cand": w.c, "k": k,
-                   "batch": "all queries in one fused launch"},
-        f"recall_at_{k}": round(rec, 4),
+                   "top_m": w.m, "n_probed_points": mp,
+                   "k": "per query {1,10,100}" if uneven else w.k,
+                   "n_probed_vectors": f"per query U[{ss.min()},{s_max}]" if uneven else w.s,
+                   "n_cand": f"per query U[{cs.min()},{c_max}]" if uneven else w.c,
+                   "batch": "all queries in one hipGraph-captured plan (projection, probe select, fused search)",
+                   "launch": {"wg_per_cu": wg, "table_log2": tl}},
+        "recall_at_10": round(rec10, 4) if rec10 is not None else None,
+        "recall_at_k": round(rec_k, 4),
         "plain_qps": round(plain_qps, 1) if plain_qps else None,
         "batched_vs_plain": round(qps / plain_qps, 1) if plain_qps else None,
         "plain_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
