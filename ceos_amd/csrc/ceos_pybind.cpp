@@ -271,6 +271,8 @@ class PyCEOs {
         return py::make_tuple(wg, t, lds);
     }
     uintptr_t handle() const { return reinterpret_cast<uintptr_t>(h_); }
+    ceos_index* raw() const { return h_; }
+    int dim() const { return d_; }
 
    private:
     ceos_index* h_ = nullptr;
@@ -281,6 +283,9 @@ class PyStream {
    public:
     explicit PyStream(int n_features) : d_(n_features) { check(ceos_stream_create(n_features, &h_)); }
     ~PyStream() { ceos_stream_destroy(h_); }
+
+    ceos_stream* raw() const { return h_; }
+    int dim() const { return d_; }
 
     PyPlan* make_plan(int nq_max, int k_max, int s_max, int c_max, bool autotune) {
         ceos_plan* p = nullptr;
@@ -367,6 +372,120 @@ class PyStream {
 
 }  // namespace
 
+// Continuous-batching server over a CEOs or streamCEOs index (ceos_server_*).
+class PyServer {
+   public:
+    PyServer(ceos_index* hi, ceos_stream* hs, int d, int max_batch, int k_max, int s_max, int c_max, int inflight,
+             int64_t hbm_budget, int64_t max_pending)
+        : d_(d), k_max_(k_max) {
+        ceos_server_config cfg{max_batch, k_max, s_max, c_max, inflight, hbm_budget, max_pending};
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = hi ? ceos_server_create(hi, &cfg, &s_) : ceos_stream_server_create(hs, &cfg, &s_);
+        }
+        check(rc);
+    }
+    ~PyServer() { close(); }
+    void close() {
+        if (!s_) return;
+        py::gil_scoped_release nogil;
+        ceos_server_destroy(s_);
+        s_ = nullptr;
+    }
+    ceos_server* live() const {
+        if (!s_) throw std::runtime_error("server is closed");
+        return s_;
+    }
+
+    int64_t submit(FArr Q, IArr k, IArr s, IArr c, py::object arrival_s) {
+        require_2d(Q, "queries");
+        const int n = (int)Q.shape(0);
+        if (k.size() != n || s.size() != n || c.size() != n)
+            throw py::value_error("k, n_probed_vectors and n_cand arrays must have one entry per query");
+        py::array_t<double, py::array::c_style | py::array::forcecast> arr;
+        const double* ap = nullptr;
+        if (!arrival_s.is_none()) {
+            arr = py::array_t<double, py::array::c_style | py::array::forcecast>(arrival_s);
+            if (arr.size() != n) throw py::value_error("arrival_s must have one entry per query");
+            ap = arr.data();
+        }
+        int64_t t0 = 0;
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_server_submit(live(), Q.data(), n, (int)Q.shape(1), k.data(), s.data(), c.data(), ap, &t0);
+        }
+        check(rc);
+        return t0;
+    }
+    py::object result(int64_t ticket, bool wait) {
+        std::vector<int32_t> ids(k_max_);
+        std::vector<float> ips(k_max_);
+        int k = 0;
+        double lat = 0.0;
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_server_result(live(), ticket, wait ? 1 : 0, ids.data(), ips.data(), &k, &lat);
+        }
+        if (rc == CEOS_EAGAIN) return py::none();
+        check(rc);
+        py::array_t<int32_t> I(k);
+        py::array_t<float> P(k);
+        std::copy(ids.begin(), ids.begin() + k, I.mutable_data());
+        std::copy(ips.begin(), ips.begin() + k, P.mutable_data());
+        return py::make_tuple(I, P, lat);
+    }
+    void update(FArr X, int n_del) {
+        require_2d(X, "new_dataset");
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_server_update(live(), X.data(), X.shape(0), (int)X.shape(1), n_del);
+        }
+        check(rc);
+    }
+    void remove(IArr ids) {
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_server_delete(live(), ids.data(), ids.size());
+        }
+        check(rc);
+    }
+    void drain() {
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_server_drain(live());
+        }
+        check(rc);
+    }
+    py::dict stats() {
+        ceos_server_stats st{};
+        check(ceos_server_get_stats(live(), &st));
+        py::dict o;
+        o["submitted"] = st.submitted;
+        o["completed"] = st.completed;
+        o["batches"] = st.batches;
+        o["mean_batch"] = st.mean_batch;
+        o["p50_ms"] = st.p50_ms;
+        o["p99_ms"] = st.p99_ms;
+        o["max_ms"] = st.max_ms;
+        o["elapsed_s"] = st.elapsed_s;
+        o["qps"] = st.qps;
+        o["inflight_slots"] = st.inflight_slots;
+        o["device_bytes"] = st.device_bytes;
+        return o;
+    }
+    void reset_stats() { check(ceos_server_reset_stats(live())); }
+
+   private:
+    ceos_server* s_ = nullptr;
+    int d_, k_max_;
+};
+
 PYBIND11_MODULE(CEOs, m) {
     m.doc() = "MI355X-native coCEOs (HIP, gfx950) with the reference CEOs module's Python API";
 
@@ -409,6 +528,31 @@ PYBIND11_MODULE(CEOs, m) {
         .def("run", &PyPlan::run, py::arg("nq"), py::arg("stream") = 0)
         .def("buffers", &PyPlan::buffers)
         .def("config", &PyPlan::config);
+
+    py::class_<PyServer>(m, "Server")
+        .def(py::init([](PyCEOs& ix, int max_batch, int k_max, int s_max, int c_max, int inflight, int64_t hbm_budget,
+                         int64_t max_pending) {
+                 return new PyServer(ix.raw(), nullptr, ix.dim(), max_batch, k_max, s_max, c_max, inflight,
+                                     hbm_budget, max_pending);
+             }),
+             py::arg("index"), py::arg("max_batch"), py::arg("k_max"), py::arg("s_max"), py::arg("c_max"),
+             py::arg("inflight") = 2, py::arg("hbm_budget") = 0, py::arg("max_pending") = 0, py::keep_alive<1, 2>())
+        .def(py::init([](PyStream& ix, int max_batch, int k_max, int s_max, int c_max, int inflight, int64_t hbm_budget,
+                         int64_t max_pending) {
+                 return new PyServer(nullptr, ix.raw(), ix.dim(), max_batch, k_max, s_max, c_max, inflight,
+                                     hbm_budget, max_pending);
+             }),
+             py::arg("index"), py::arg("max_batch"), py::arg("k_max"), py::arg("s_max"), py::arg("c_max"),
+             py::arg("inflight") = 2, py::arg("hbm_budget") = 0, py::arg("max_pending") = 0, py::keep_alive<1, 2>())
+        .def("submit", &PyServer::submit, py::arg("queries"), py::arg("n_neighbors"), py::arg("n_probed_vectors"),
+             py::arg("n_cand"), py::arg("arrival_s") = py::none())
+        .def("result", &PyServer::result, py::arg("ticket"), py::arg("wait") = true)
+        .def("update", &PyServer::update, py::arg("new_dataset"), py::arg("n_delPoints") = 0)
+        .def("delete_ids", &PyServer::remove, py::arg("ids"))
+        .def("drain", &PyServer::drain)
+        .def("stats", &PyServer::stats)
+        .def("reset_stats", &PyServer::reset_stats)
+        .def("close", &PyServer::close);
 
     py::class_<PyStream>(m, "streamCEOs")
         .def(py::init<int>(), py::arg("n_features"))

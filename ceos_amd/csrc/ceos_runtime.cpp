@@ -1,630 +1,6 @@
-// Host runtime of libceos_amd: device-resident coCEOs / CEOs-Est / streamCEOs indexes
-// behind the C ABI declared in include/ceos_amd.h.
-//
-// Each index owns one HIP stream, its points (HBM copy of X), the sign diagonals,
-// the bucket lists and the query scratch. Calls on one index are serialised by a
-// mutex; different indexes run independently.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <array>
-#include <chrono>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <mutex>
-#include <stdexcept>
-#include <string>
-#include <vector>
-
-#include "../../include/ceos_amd.h"
-#include "ceos_kernels.h"
-
-using ceos::Entry;
-using ceos::IndexDims;
-
-namespace {
-
-thread_local std::string g_err;
-
-struct ArgError : std::runtime_error { using std::runtime_error::runtime_error; };
-struct DevError : std::runtime_error { using std::runtime_error::runtime_error; };
-struct MemError : std::runtime_error { using std::runtime_error::runtime_error; };
-struct StateError : std::runtime_error { using std::runtime_error::runtime_error; };
-
-#define HIP_CHECK(expr)                                                                         \
-    do {                                                                                        \
-        hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess)                                                                   \
-            throw DevError(std::string(#expr) + " failed: " + hipGetErrorString(e_));           \
-    } while (0)
-
-template <typename F>
-int guarded(F&& f) {
-    try {
-        f();
-        g_err.clear();
-        return CEOS_OK;
-    } catch (const ArgError& e) {
-        g_err = e.what();
-        return CEOS_EINVAL;
-    } catch (const MemError& e) {
-        g_err = e.what();
-        return CEOS_ENOMEM;
-    } catch (const StateError& e) {
-        g_err = e.what();
-        return CEOS_ESTATE;
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return CEOS_EDEVICE;
-    }
-}
-
-template <typename T>
-struct DBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    explicit DBuf(size_t count) { alloc(count); }
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-    DBuf& operator=(DBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
-        return *this;
-    }
-    ~DBuf() { release(); }
-    void alloc(size_t count) {
-        release();
-        if (count == 0) return;
-        if (hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
-            throw MemError("device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
-        }
-        n = count;
-    }
-    void ensure(size_t count) {
-        if (count > n) alloc(count);
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
-// src/CEOs.h:76-79
-int fht_dim(int n_proj, int d) {
-    if (n_proj < d) return 1 << (int)std::ceil(std::log2((double)d));
-    return 1 << (int)std::ceil(std::log2((double)n_proj));
-}
-
-// Sign bits of src/Utilities.cpp:21-41: std::default_random_engine (minstd_rand0,
-// x <- 16807 x mod (2^31-1)) drawn through libstdc++'s uniform_int_distribution(0,1),
-// which rejects x-1 >= 2147483644 and returns (x-1) / 1073741822.
-std::vector<uint8_t> sign_bits(int nbits, unsigned seed) {
-    uint64_t x = seed % 2147483647u;
-    if (x == 0) x = 1;
-    std::vector<uint8_t> bits(nbits);
-    for (int i = 0; i < nbits; ++i) {
-        uint64_t r;
-        do {
-            x = (x * 16807u) % 2147483647u;
-            r = x - 1;
-        } while (r >= 2147483644u);
-        bits[i] = (uint8_t)(r / 1073741822u);
-    }
-    return bits;
-}
-
-int device_cus() {
-    static int cus = [] {
-        int dev = 0, v = 0;
-        if (hipGetDevice(&dev) != hipSuccess) return 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-        return v;
-    }();
-    if (cus <= 0) throw DevError("no HIP device available");
-    return cus;
-}
-
-constexpr int kMaxLdsBytes = 160 * 1024 - 2048;  // minus the kernels' static LDS
-constexpr int kMaxTopM = 8192;  // bucket depth; searches read at most kMaxProbedPoints of it
-constexpr int kMaxProbes = 4096;
-constexpr int kMaxK = 4096;
-
-// Device state shared by CEOs and streamCEOs.
-// Device scratch of the coCEOs search for one launch configuration.
-struct ScratchSet {
-    DBuf<uint64_t> cand;
-    DBuf<float> qproj;
-    DBuf<uint2> pairs;
-    DBuf<uint4> stash;
-    DBuf<uint32_t> tkeys;
-    DBuf<int32_t> probes;
-    DBuf<int32_t> work, status;
-
-    ceos::QueryScratch ensure(const ceos::QueryConfig& cfg, int nq, int DR) {
-        cand.ensure((size_t)cfg.grid * cfg.cand_stride);
-        pairs.ensure((size_t)cfg.grid * cfg.cand_stride);
-        stash.ensure((size_t)cfg.grid * ceos::kSearchStashList);
-        tkeys.ensure((size_t)cfg.grid << cfg.table_log2);
-        qproj.ensure((size_t)std::max(nq, 1) * DR);
-        probes.ensure((size_t)std::max(nq, 1) * cfg.s_max);
-        if (!work.p) work.alloc(1);
-        if (!status.p) {
-            status.alloc(1);
-            HIP_CHECK(hipMemset(status.p, 0, 4));
-        }
-        return ceos::QueryScratch{qproj.p, probes.p, cand.p, pairs.p, stash.p, tkeys.p, work.p, status.p};
-    }
-    void release() {
-        cand.release(); qproj.release(); pairs.release(); stash.release(); tkeys.release(); probes.release();
-    }
-};
-
-struct Core {
-    int d = 0;
-    int D = 256, R = 1, m = 100, F = 0, L = 0;
-    int seed = -1;
-    unsigned seed_used = 0;
-    int threads = 1;
-    int s = 10, mp = 10, c = 10;  // n_probed_vectors, n_probed_points, n_cand
-    bool centering = false;
-    bool params_set = false;
-    bool built = false;
-    hipStream_t st_ = nullptr;
-
-    std::vector<uint8_t> bits;
-    DBuf<uint32_t> signs;
-    DBuf<Entry> bkt;
-    DBuf<int32_t> bsize;
-    // stored points: ring of `cap` rows, logical id 0 at slot `head`, absolute id `base`
-    DBuf<float> X;
-    int64_t cap = 0, head = 0;
-    uint32_t base = 0, live = 0;
-    DBuf<uint32_t> dead;  // stream random-delete bitmap over ring slots
-
-    // query scratch
-    ScratchSet scr;
-    int wg_per_cu = 0, table_log2 = 0;
-    int last_lds = 0, last_tlog2 = 0, last_wg = 0;
-    // optional per-launch kernel timing (HIP events on the launch stream)
-    bool ktiming = false;
-    std::vector<std::array<hipEvent_t, 3>> kev;
-    size_t kev_used = 0;
-
-    // CEOs-Est projections [DR][n]
-    DBuf<float> estP;
-    int64_t est_n = 0;
-
-    std::mutex mu;
-
-    Core() = default;
-    ~Core() {
-        for (auto& e : kev)
-            for (hipEvent_t x : e) (void)hipEventDestroy(x);
-        if (st_) (void)hipStreamDestroy(st_);
-    }
-    static constexpr size_t kMaxTimedLaunches = 4096;
-    // Attaches the next event triple to a search launch when timing is on.
-    void arm_timing(ceos::QueryConfig& cfg) {
-        cfg.ev[0] = cfg.ev[1] = cfg.ev[2] = nullptr;
-        if (!ktiming || kev_used >= kMaxTimedLaunches) return;
-        if (kev_used == kev.size()) {
-            std::array<hipEvent_t, 3> e{};
-            for (auto& x : e) HIP_CHECK(hipEventCreate(&x));
-            kev.push_back(e);
-        }
-        for (int i = 0; i < 3; ++i) cfg.ev[i] = kev[kev_used][i];
-        ++kev_used;
-    }
-    // The stream is created on first device use, so handles can be made and
-    // configured on a host without a GPU.
-    hipStream_t st() {
-        if (!st_) HIP_CHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-        return st_;
-    }
-
-    int DR() const { return D * R; }
-    IndexDims dims() const { return IndexDims{d, L, D, R, m}; }
-    hipStream_t stream(void* s_) { return s_ ? reinterpret_cast<hipStream_t>(s_) : st(); }
-
-    void set_index_param(int n_proj, int n_repeats, int top_m, int n_threads, int seed_) {
-        if (n_proj <= 0 || n_repeats <= 0 || top_m <= 0)
-            throw ArgError("n_proj, n_repeats and top_m must be positive");
-        const int f = fht_dim(n_proj, d);
-        if (f > 4096) throw ArgError("fhtDim " + std::to_string(f) + " exceeds 4096 (n_proj and n_features <= 4096)");
-        if (top_m > kMaxTopM) throw ArgError("top_m above " + std::to_string(kMaxTopM) + " is not supported");
-        D = n_proj; R = n_repeats; m = top_m; seed = seed_;
-        threads = n_threads;
-        F = f;
-        L = 0;
-        while ((1 << L) < F) ++L;
-        params_set = true;
-        built = false;
-    }
-
-    void clear() {
-        bkt.release(); bsize.release(); X.release(); signs.release(); dead.release();
-        scr.release(); estP.release();
-        bits.clear();
-        cap = head = 0; base = live = 0; est_n = 0;
-        built = false;
-    }
-
-    void make_signs() {
-        seed_used = seed >= 0 ? (unsigned)seed
-                              : (unsigned)std::chrono::system_clock::now().time_since_epoch().count();
-        bits = sign_bits(F * 3 * R, seed_used);
-        const int W = (F + 31) / 32;
-        std::vector<uint32_t> words((size_t)W * 3 * R, 0u);
-        for (int r = 0; r < R; ++r)
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < F; ++j)
-                    if (bits[(size_t)r * 3 * F + (size_t)i * F + j])
-                        words[(size_t)(r * 3 + i) * W + j / 32] |= 1u << (j % 32);
-        signs.alloc(words.size());
-        HIP_CHECK(hipMemcpyAsync(signs.p, words.data(), words.size() * 4, hipMemcpyHostToDevice, st()));
-        HIP_CHECK(hipStreamSynchronize(st()));
-    }
-
-    // Copies n rows (host or device pointer) into a fresh ring starting at slot 0.
-    void load_points(const float* src, int64_t n, int64_t capacity, bool src_device, hipStream_t s_) {
-        X.alloc((size_t)capacity * d);
-        cap = capacity; head = 0; base = 0; live = (uint32_t)n;
-        HIP_CHECK(hipMemcpyAsync(X.p, src, (size_t)n * d * 4,
-                                 src_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s_));
-    }
-
-    // coCEOs buckets over rows 0..n-1 of X (ring slots 0..n-1): see ceos_build.hip.
-    void build_buckets(int64_t n, hipStream_t s_) {
-        const int DR_ = DR();
-        const int nl = 2 * DR_;
-        const IndexDims dm = dims();
-        if (m > n) throw ArgError("top_m must not exceed the number of points");
-
-        int64_t S_target = std::max<int64_t>(65536, (64 * n + m - 1) / m);
-        S_target = std::min<int64_t>(S_target, (int64_t(4) << 30) / (4LL * DR_));
-        S_target = std::max<int64_t>(S_target, 1);
-        const int64_t stride = std::max<int64_t>(1, n / S_target);
-        const int64_t S = (n + stride - 1) / stride;
-        const bool exact = stride == 1;
-        const int rank = exact ? m : (int)std::min<int64_t>(S, (2 * (int64_t)m * S + n - 1) / n + 8);
-
-        DBuf<float> thr(nl);
-        {
-            DBuf<float> sample((size_t)DR_ * S);
-            HIP_CHECK(ceos::launch_project_rows(dm, X.p, 0, stride, S, signs.p, sample.p, S, true, s_));
-            HIP_CHECK(ceos::launch_sample_thresholds(sample.p, S, DR_, rank, thr.p, s_));
-            HIP_CHECK(hipStreamSynchronize(s_));
-        }
-        std::vector<int64_t> hbase(nl);
-        std::vector<int32_t> hcap(nl, (int32_t)std::min<int64_t>(n, 4LL * m + 256));
-        std::vector<int32_t> hcount(nl);
-        std::vector<float> hthr;
-        DBuf<int64_t> dbase(nl);
-        DBuf<int32_t> dcap(nl), dcount(nl);
-        DBuf<Entry> ent;
-        for (int iter = 0;; ++iter) {
-            int64_t total = 0;
-            for (int l = 0; l < nl; ++l) { hbase[l] = total; total += hcap[l]; }
-            ent.alloc((size_t)total);
-            HIP_CHECK(hipMemcpyAsync(dbase.p, hbase.data(), nl * 8, hipMemcpyHostToDevice, s_));
-            HIP_CHECK(hipMemcpyAsync(dcap.p, hcap.data(), nl * 4, hipMemcpyHostToDevice, s_));
-            HIP_CHECK(ceos::launch_fill_i32(dcount.p, 0, nl, s_));
-            HIP_CHECK(ceos::launch_scan_append(dm, X.p, n, signs.p, thr.p, nullptr, dbase.p, dcap.p, dcount.p,
-                                               ent.p, s_));
-            HIP_CHECK(hipMemcpyAsync(hcount.data(), dcount.p, nl * 4, hipMemcpyDeviceToHost, s_));
-            HIP_CHECK(hipStreamSynchronize(s_));
-            bool redo = false;
-            for (int l = 0; l < nl; ++l) {
-                if (hcount[l] < m) {
-                    if (hthr.empty()) {
-                        hthr.resize(nl);
-                        HIP_CHECK(hipMemcpy(hthr.data(), thr.p, nl * 4, hipMemcpyDeviceToHost));
-                    }
-                    hthr[l] = -INFINITY;  // accept every point for this list
-                    hcap[l] = (int32_t)n;
-                    redo = true;
-                } else if (hcount[l] > hcap[l]) {
-                    hcap[l] = hcount[l];
-                    redo = true;
-                }
-            }
-            if (!redo) break;
-            if (iter >= 2) throw DevError("bucket construction did not converge");
-            if (!hthr.empty()) HIP_CHECK(hipMemcpy(thr.p, hthr.data(), nl * 4, hipMemcpyHostToDevice));
-        }
-        bkt.alloc((size_t)nl * m);
-        bsize.alloc(nl);
-        HIP_CHECK(ceos::launch_list_select(nl, m, dbase.p, dcap.p, dcount.p, ent.p, bkt.p, bsize.p, s_));
-        HIP_CHECK(hipStreamSynchronize(s_));
-    }
-
-    // Picks the LDS score-tile size and grid for a search with the given bounds:
-    // the largest tile that fits the per-workgroup LDS share (wg_per_cu workgroups
-    // of 512 threads per CU; default 1, i.e. a 32K-slot tile of 4-byte ids).
-    ceos::QueryConfig search_config(int s_max, int c_max, int k_max) {
-        ceos::QueryConfig cfg = search_config_with(s_max, c_max, k_max, wg_per_cu, table_log2);
-        arm_timing(cfg);
-        return cfg;
-    }
-    // The same for an explicit launch shape (0 = automatic), without kernel timing.
-    ceos::QueryConfig search_config_with(int s_max, int c_max, int k_max, int wg_req, int tl_req) {
-        const int wg = wg_req > 0 ? wg_req : 1;
-        const int budget = std::min(kMaxLdsBytes, (160 * 1024) / wg - 2048);
-        int t = 0;
-        if (tl_req > 0) {
-            t = tl_req;
-        } else {
-            for (int cand_t = 15; cand_t >= 8; --cand_t)
-                if (ceos::search_lds_bytes(d, cand_t, s_max, c_max, k_max) <= budget) { t = cand_t; break; }
-            if (t == 0) t = 8;
-        }
-        const int lds = ceos::search_lds_bytes(d, t, s_max, c_max, k_max);
-        if (lds > kMaxLdsBytes)
-            throw ArgError("search parameters need " + std::to_string(lds) +
-                           " bytes of LDS per workgroup (n_cand or n_probed_vectors too large)");
-        if (std::min(mp, m) > ceos::kMaxProbedPoints)
-            throw ArgError("n_probed_points above " + std::to_string(ceos::kMaxProbedPoints) + " is not supported");
-        if ((int64_t)s_max * std::min(mp, m) > (int64_t(1) << 26))
-            throw ArgError("n_probed_vectors * n_probed_points above 2^26 is not supported");
-        ceos::QueryConfig cfg{};
-        cfg.table_log2 = t;
-        cfg.lds_bytes = lds;
-        const int fit = std::max(1, (160 * 1024) / (lds + 2048));
-        cfg.grid = device_cus() * std::min(wg, fit);
-        cfg.s_max = s_max; cfg.c_max = c_max; cfg.k_max = k_max;
-        cfg.cand_stride = (int64_t)s_max * std::max(1, std::min(mp, m));
-        last_lds = lds; last_tlog2 = t; last_wg = std::min(wg, fit);
-        return cfg;
-    }
-
-    ceos::QueryScratch scratch(const ceos::QueryConfig& cfg, int nq) { return scr.ensure(cfg, nq, DR()); }
-
-    ceos::RowRing rows() const { return ceos::RowRing{X.p, cap, head, base, live}; }
-
-    void check_status() {
-        int32_t flag = 0;
-        HIP_CHECK(hipMemcpy(&flag, scr.status.p, 4, hipMemcpyDeviceToHost));
-        if (flag) {
-            HIP_CHECK(hipMemset(scr.status.p, 0, 4));
-            throw DevError("LDS score tile overflowed; rerun with a larger table (ceos_set_search_config)");
-        }
-    }
-
-    void validate_query(int s_, int mp_, int c_, int k, int s_limit, const char* s_limit_name) const {
-        if (!built) throw StateError("index is not built");
-        if (mp_ > m) throw ArgError("Number of probed points must be smaller than number of indexed top-m points");
-        if (s_ > s_limit)
-            throw ArgError(std::string("Number of probed vectors must be smaller than ") + s_limit_name);
-        if (s_ <= 0 || mp_ <= 0 || c_ <= 0) throw ArgError("n_probed_vectors, n_probed_points and n_cand must be positive");
-        if (k < 0 || k > kMaxK) throw ArgError("n_neighbors must be in [0, " + std::to_string(kMaxK) + "]");
-        if (s_ > kMaxProbes) throw ArgError("n_probed_vectors above " + std::to_string(kMaxProbes) + " is not supported");
-    }
-
-    // Uniform-parameter coCEOs search on device pointers.
-    void search_device(const float* dQ, int nq, int k, int s_, int c_, int32_t* ids, float* ips, hipStream_t s__) {
-        if (nq <= 0 || k == 0) return;
-        const ceos::QueryConfig cfg = search_config(s_, c_, k);
-        const ceos::QueryScratch scr = scratch(cfg, nq);
-        ceos::QueryBatch qb{dQ, nq, nullptr, nullptr, nullptr, k, s_, c_, mp, nullptr, ids, ips};
-        HIP_CHECK(ceos::launch_coceos_search(dims(), signs.p, bkt.p, bsize.p, rows(), qb, cfg, scr, s__));
-    }
-
-    void search_host(const float* Q, int nq, int k, int s_, int c_, int32_t* ids, float* ips) {
-        if (nq <= 0 || k == 0) return;
-        DBuf<float> dQ((size_t)nq * d);
-        DBuf<int32_t> dI((size_t)nq * k);
-        DBuf<float> dP((size_t)nq * k);
-        HIP_CHECK(hipMemcpyAsync(dQ.p, Q, (size_t)nq * d * 4, hipMemcpyHostToDevice, st()));
-        search_device(dQ.p, nq, k, s_, c_, dI.p, dP.p, st());
-        HIP_CHECK(hipMemcpyAsync(ids, dI.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, st()));
-        HIP_CHECK(hipMemcpyAsync(ips, dP.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, st()));
-        HIP_CHECK(hipStreamSynchronize(st()));
-        check_status();
-    }
-
-    void get_buckets(int32_t* cid, float* cval, int32_t* fid, float* fval, int32_t* csz, int32_t* fsz) {
-        if (!built) throw StateError("index is not built");
-        const int DR_ = DR();
-        std::vector<Entry> h((size_t)2 * DR_ * m);
-        std::vector<int32_t> sz(2 * DR_);
-        HIP_CHECK(hipMemcpy(h.data(), bkt.p, h.size() * sizeof(Entry), hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(sz.data(), bsize.p, sz.size() * 4, hipMemcpyDeviceToHost));
-        for (int col = 0; col < DR_; ++col) {
-            for (int side = 0; side < 2; ++side) {
-                const int l = 2 * col + side;
-                int32_t* oid = side ? fid : cid;
-                float* ov = side ? fval : cval;
-                for (int t = 0; t < m; ++t) {
-                    const Entry e = h[(size_t)l * m + t];
-                    const bool valid = t < sz[l];
-                    oid[(size_t)col * m + t] = valid ? (int32_t)(e.x - base) : 0;
-                    uint32_t bits_ = valid ? e.y : 0u;
-                    std::memcpy(&ov[(size_t)col * m + t], &bits_, 4);
-                }
-                if (side == 0 && csz) csz[col] = sz[l];
-                if (side == 1 && fsz) fsz[col] = sz[l];
-            }
-        }
-    }
-};
-
-void check_matrix(const void* p, int64_t n, int d, int expect_d) {
-    if (!p && n > 0) throw ArgError("null data pointer");
-    if (n < 0) throw ArgError("negative row count");
-    if (d != expect_d)
-        throw ArgError("feature dimension " + std::to_string(d) + " does not match the index (" +
-                       std::to_string(expect_d) + ")");
-}
-
-}  // namespace
-
-struct ceos_index {
-    Core core;
-    int n_points = 0;
-};
-
-struct ceos_stream {
-    Core core;
-};
-
-// A batched search captured in a hipGraph over plan-owned buffers (see ceos_plan_create).
-struct ceos_plan {
-    Core* core = nullptr;
-    bool stream_index = false;  // streamCEOs: n_probed_vectors is bounded by n_proj
-    int nq_max = 0, k_max = 0, s_max = 0, c_max = 0, flags = 0;
-    DBuf<float> Q, ips;
-    DBuf<int32_t> k, s, c, ids, nq_dev;
-    ScratchSet scr;
-    ceos::QueryConfig cfg{};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipStream_t cap = nullptr;
-    // everything the captured launches hold by value
-    struct Snap {
-        const void *signs, *bkt, *bsize, *X;
-        int64_t cap, head;
-        uint32_t base, live;
-        int mp, m, D, R, L, d;
-        bool operator==(const Snap& o) const {
-            return signs == o.signs && bkt == o.bkt && bsize == o.bsize && X == o.X && cap == o.cap &&
-                   head == o.head && base == o.base && live == o.live && mp == o.mp && m == o.m && D == o.D &&
-                   R == o.R && L == o.L && d == o.d;
-        }
-    } snap{};
-    bool tuned = false;
-    double tuned_ms = 0.0;
-    int wg = 0, tl = 0;
-
-    ~ceos_plan() {
-        drop();
-        if (cap) (void)hipStreamDestroy(cap);
-    }
-    void drop() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
-    }
-    Snap current() const {
-        const Core& x = *core;
-        return Snap{x.signs.p, x.bkt.p, x.bsize.p, x.X.p, x.cap, x.head, x.base, x.live, x.mp, x.m, x.D, x.R, x.L, x.d};
-    }
-    void capture(int wg_, int tl_) {
-        drop();
-        Core& x = *core;
-        cfg = x.search_config_with(s_max, c_max, k_max, wg_, tl_);
-        const ceos::QueryScratch sc = scr.ensure(cfg, nq_max, x.DR());
-        // k0 = k_max: rows at q * k_max, zero padded to k_max
-        ceos::QueryBatch qb{Q.p, nq_max, k.p, s.p, c.p, k_max, 0, 0, x.mp, nullptr, ids.p, ips.p, nq_dev.p};
-        if (!cap) HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-        HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        const hipError_t e1 =
-            ceos::launch_coceos_search(x.dims(), x.signs.p, x.bkt.p, x.bsize.p, x.rows(), qb, cfg, sc, cap);
-        hipGraph_t g = nullptr;
-        const hipError_t e2 = hipStreamEndCapture(cap, &g);
-        if (e1 != hipSuccess || e2 != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            throw DevError(std::string("search plan capture failed: ") +
-                           hipGetErrorString(e1 != hipSuccess ? e1 : e2));
-        }
-        graph = g;
-        HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        snap = current();
-        wg = wg_;
-        tl = tl_;
-    }
-    void launch(int nq, hipStream_t st) {
-        HIP_CHECK(ceos::launch_fill_i32(nq_dev.p, nq, 1, st));  // a kernel, outside the graph
-        HIP_CHECK(hipGraphLaunch(exec, st));
-    }
-    // Times each candidate launch shape once on this batch and keeps the fastest. Every
-    // candidate computes the same results, so the batch's outputs are valid afterwards.
-    void tune(int nq, hipStream_t st) {
-        Core& x = *core;
-        const int t1 = x.search_config_with(s_max, c_max, k_max, 1, 0).table_log2;
-        const int t2 = x.search_config_with(s_max, c_max, k_max, 2, 0).table_log2;
-        std::vector<std::pair<int, int>> cands = {{1, 0}, {2, 0}};
-        if (t1 - 1 >= 8) cands.push_back({1, t1 - 1});
-        if (t2 - 1 >= 8) cands.push_back({2, t2 - 1});
-        hipEvent_t e0, e1;
-        HIP_CHECK(hipEventCreate(&e0));
-        HIP_CHECK(hipEventCreate(&e1));
-        double best = 1e30;
-        std::pair<int, int> pick = cands[0];
-        try {
-            for (const auto& cd : cands) {
-                capture(cd.first, cd.second);
-                launch(nq, st);  // warm
-                HIP_CHECK(hipEventRecord(e0, st));
-                launch(nq, st);
-                HIP_CHECK(hipEventRecord(e1, st));
-                HIP_CHECK(hipEventSynchronize(e1));
-                float ms = 0.0f;
-                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) {
-                    best = ms;
-                    pick = cd;
-                }
-            }
-        } catch (...) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            throw;
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        capture(pick.first, pick.second);
-        tuned = true;
-        tuned_ms = best;
-    }
-    void validate() const {
-        const Core& x = *core;
-        x.validate_query(s_max, x.mp, c_max, k_max, stream_index ? x.D : x.DR(),
-                         stream_index ? "number of projections" : "n_proj * n_repeats");
-    }
-    void run(int nq, hipStream_t st) {
-        if (nq < 0 || nq > nq_max) throw ArgError("batch of " + std::to_string(nq) + " exceeds the plan's nq_max");
-        validate();
-        if (nq == 0) return;
-        if (!exec || !(snap == current())) capture(wg, tl);
-        if ((flags & CEOS_PLAN_AUTOTUNE) && !tuned)
-            tune(nq, st);
-        else
-            launch(nq, st);
-    }
-};
-
-namespace {
-int make_plan(Core& x, bool stream_index, int nq_max, int k_max, int s_max, int c_max, int flags, ceos_plan** out) {
-    return guarded([&] {
-        if (!out) throw ArgError("null output handle");
-        if (nq_max <= 0 || k_max <= 0 || s_max <= 0 || c_max <= 0)
-            throw ArgError("nq_max, k_max, s_max and c_max must be positive");
-        std::lock_guard<std::mutex> g(x.mu);
-        auto* p = new ceos_plan();
-        try {
-            p->core = &x;
-            p->stream_index = stream_index;
-            p->nq_max = nq_max; p->k_max = k_max; p->s_max = s_max; p->c_max = c_max; p->flags = flags;
-            if (x.built) p->validate();
-            p->Q.alloc((size_t)nq_max * x.d);
-            p->k.alloc(nq_max); p->s.alloc(nq_max); p->c.alloc(nq_max);
-            p->ids.alloc((size_t)nq_max * k_max);
-            p->ips.alloc((size_t)nq_max * k_max);
-            p->nq_dev.alloc(1);
-        } catch (...) {
-            delete p;
-            throw;
-        }
-        *out = p;
-    });
-}
-}  // namespace
+// C-ABI entry points of libceos_amd (include/ceos_amd.h): device-resident coCEOs,
+// CEOs-Est and streamCEOs indexes, captured search plans.
+#include "ceos_core.h"
 
 extern "C" {
 

@@ -1,0 +1,523 @@
+// Serving engine of libceos_amd: continuous batching of coCEOs queries over a resident
+// index (include/ceos_amd.h, "serving"). No reference counterpart: the reference only
+// searches a whole query matrix per call (src/CEOs.cpp:450-694).
+//
+// One worker thread owns the device side. Requests (one query each, with its own k,
+// n_probed_vectors and n_cand) wait in an arrival-ordered queue. Whenever one of the
+// server's `inflight` batch slots is free, the worker takes every request that has
+// arrived (up to max_batch) into that slot: pinned host staging -> device buffers of a
+// captured search plan -> plan replay -> results back, all on the slot's own stream,
+// ending in an event. Requests that arrive while batches run are admitted into the
+// next free slot, so batches overlap on the device (the next persistent grid fills
+// the CUs as the previous one drains) and with the host copies. The number of slots
+// is bounded by the HBM budget: each slot holds one plan's buffers and scratch.
+// streamCEOs updates and deletes are applied between batches: the worker waits for
+// the batches in flight, applies them in submission order, and the next batch sees
+// the new index.
+#include <condition_variable>
+#include <deque>
+#include <queue>
+#include <thread>
+#include <unordered_map>
+
+#include "ceos_core.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    std::vector<float> q;
+    int k = 0, s = 0, c = 0;
+    Clock::time_point arrival;
+    bool done = false;
+    bool failed = false;
+    std::vector<int32_t> ids;
+    std::vector<float> ips;
+    double latency_ms = 0.0;
+};
+
+struct Update {
+    bool is_delete = false;
+    std::vector<float> X;  // new rows (insert)
+    int64_t n_new = 0;
+    int n_del = 0;
+    std::vector<int32_t> ids;  // random delete
+};
+
+// Pinned host memory (hipHostMalloc).
+template <typename T>
+struct HBuf {
+    T* p = nullptr;
+    void alloc(size_t n) {
+        if (hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(1, n) * sizeof(T), hipHostMallocDefault) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            throw MemError("pinned host allocation failed");
+        }
+    }
+    ~HBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+struct Slot {
+    ceos_plan* plan = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    HBuf<float> hQ, hP;
+    HBuf<int32_t> hk, hs, hc, hI;
+    std::vector<int64_t> tickets;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct ceos_server {
+    Core* core = nullptr;
+    ceos_index* index = nullptr;
+    ceos_stream* stream = nullptr;
+    ceos_server_config cfg{};
+    std::vector<std::unique_ptr<Slot>> slots;
+    int64_t device_bytes = 0;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    // (arrival, ticket), earliest first
+    std::priority_queue<std::pair<Clock::time_point, int64_t>, std::vector<std::pair<Clock::time_point, int64_t>>,
+                        std::greater<>>
+        pending;
+    std::unordered_map<int64_t, Request> reqs;
+    std::deque<Update> updates;
+    int64_t next_ticket = 1;
+    bool stop = false;
+    std::string worker_error;  // fatal: the worker stopped
+    std::string update_error;  // the last failed update (reported by ceos_server_drain)
+    std::thread worker;
+
+    // statistics
+    int64_t submitted = 0, completed = 0, batches = 0, batched_queries = 0;
+    std::vector<double> lat_ms;
+    Clock::time_point first_arrival{}, last_done{};
+    bool any_arrival = false;
+
+    ~ceos_server() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv_work.notify_all();
+        if (worker.joinable()) worker.join();
+        for (auto& sl : slots) {
+            if (sl->plan) ceos_plan_destroy(sl->plan);
+            if (sl->ev) (void)hipEventDestroy(sl->ev);
+            if (sl->st) (void)hipStreamDestroy(sl->st);
+        }
+    }
+
+    int k_max() const { return cfg.k_max; }
+
+    // Moves a finished slot's results to its requests (mu held).
+    void reap(Slot& sl) {
+        const auto now = Clock::now();
+        for (size_t i = 0; i < sl.tickets.size(); ++i) {
+            auto it = reqs.find(sl.tickets[i]);
+            if (it == reqs.end()) continue;
+            Request& r = it->second;
+            r.ids.assign(sl.hI.p + i * cfg.k_max, sl.hI.p + i * cfg.k_max + r.k);
+            r.ips.assign(sl.hP.p + i * cfg.k_max, sl.hP.p + i * cfg.k_max + r.k);
+            r.latency_ms = std::chrono::duration<double, std::milli>(now - r.arrival).count();
+            r.done = true;
+            lat_ms.push_back(r.latency_ms);
+            ++completed;
+        }
+        last_done = now;
+        sl.tickets.clear();
+        sl.busy = false;
+        cv_done.notify_all();
+    }
+
+    void fail_all(const std::string& msg) {  // mu held; the server stops serving
+        worker_error = msg;
+        while (!pending.empty()) pending.pop();
+        for (auto& kv : reqs)
+            if (!kv.second.done) {
+                kv.second.done = true;
+                kv.second.failed = true;
+            }
+        cv_done.notify_all();
+    }
+
+    // Launches one batch of arrived requests into a free slot (mu held, released
+    // around the device calls).
+    void launch(Slot& sl, std::unique_lock<std::mutex>& lk) {
+        const auto now = Clock::now();
+        const int d = core->d;
+        int nb = 0;
+        while (!pending.empty() && nb < cfg.max_batch && pending.top().first <= now) {
+            const int64_t t = pending.top().second;
+            pending.pop();
+            Request& r = reqs.at(t);
+            std::memcpy(sl.hQ.p + (size_t)nb * d, r.q.data(), (size_t)d * 4);
+            sl.hk.p[nb] = r.k;
+            sl.hs.p[nb] = r.s;
+            sl.hc.p[nb] = r.c;
+            r.q.clear();
+            r.q.shrink_to_fit();
+            sl.tickets.push_back(t);
+            ++nb;
+        }
+        if (nb == 0) return;
+        sl.busy = true;
+        ++batches;
+        batched_queries += nb;
+        lk.unlock();
+        try {
+            float* dQ;
+            int32_t *dk, *ds, *dc, *dI;
+            float* dP;
+            HIP_CHECK(ceos_plan_buffers(sl.plan, &dQ, &dk, &ds, &dc, &dI, &dP) == CEOS_OK ? hipSuccess
+                                                                                            : hipErrorInvalidValue);
+            HIP_CHECK(hipMemcpyAsync(dQ, sl.hQ.p, (size_t)nb * d * 4, hipMemcpyHostToDevice, sl.st));
+            HIP_CHECK(hipMemcpyAsync(dk, sl.hk.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl.st));
+            HIP_CHECK(hipMemcpyAsync(ds, sl.hs.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl.st));
+            HIP_CHECK(hipMemcpyAsync(dc, sl.hc.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl.st));
+            if (ceos_plan_run(sl.plan, nb, sl.st) != CEOS_OK) throw DevError(g_err);
+            HIP_CHECK(hipMemcpyAsync(sl.hI.p, dI, (size_t)nb * cfg.k_max * 4, hipMemcpyDeviceToHost, sl.st));
+            HIP_CHECK(hipMemcpyAsync(sl.hP.p, dP, (size_t)nb * cfg.k_max * 4, hipMemcpyDeviceToHost, sl.st));
+            HIP_CHECK(hipEventRecord(sl.ev, sl.st));
+        } catch (const std::exception& e) {
+            lk.lock();
+            sl.busy = false;
+            fail_all(e.what());
+            return;
+        }
+        lk.lock();
+    }
+
+    void apply_update(Update& u, std::unique_lock<std::mutex>& lk) {
+        lk.unlock();
+        int rc;
+        if (u.is_delete)
+            rc = ceos_stream_delete_ids(stream, u.ids.data(), (int64_t)u.ids.size());
+        else
+            rc = ceos_stream_update(stream, u.X.data(), u.n_new, core->d, u.n_del);
+        const std::string err = rc == CEOS_OK ? std::string() : g_err;
+        lk.lock();
+        if (rc != CEOS_OK) update_error = "stream update failed: " + err;  // searches go on
+        cv_done.notify_all();
+    }
+
+    void run() {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            // reap finished batches
+            for (auto& sl : slots) {
+                if (!sl->busy) continue;
+                const hipError_t q = hipEventQuery(sl->ev);
+                if (q == hipSuccess) {
+                    reap(*sl);
+                } else if (q == hipErrorNotReady) {
+                    (void)hipGetLastError();  // not an error; keep it out of later launch checks
+                } else {
+                    fail_all(std::string("batch failed: ") + hipGetErrorString(q));
+                    return;
+                }
+            }
+            bool any_busy = false;
+            for (auto& sl : slots) any_busy |= sl->busy;
+            if (stop && pending.empty() && updates.empty() && !any_busy) break;
+            // updates: between batches
+            if (!updates.empty()) {
+                if (any_busy) {
+                    for (auto& sl : slots)
+                        if (sl->busy) {
+                            hipEvent_t ev = sl->ev;
+                            lk.unlock();
+                            (void)hipEventSynchronize(ev);
+                            lk.lock();
+                            reap(*sl);
+                        }
+                    continue;
+                }
+                Update u = std::move(updates.front());
+                updates.pop_front();
+                apply_update(u, lk);
+                continue;
+            }
+            // launch into a free slot when a request has arrived
+            const auto now = Clock::now();
+            const bool ready = !pending.empty() && pending.top().first <= now;
+            Slot* free_slot = nullptr;
+            for (auto& sl : slots)
+                if (!sl->busy) {
+                    free_slot = sl.get();
+                    break;
+                }
+            if (ready && free_slot) {
+                launch(*free_slot, lk);
+                continue;
+            }
+            // wait: for a batch to finish (poll), an arrival, or a submission
+            if (any_busy) {
+                cv_work.wait_for(lk, std::chrono::microseconds(20));
+            } else if (!pending.empty()) {
+                cv_work.wait_until(lk, pending.top().first);
+            } else {
+                cv_work.wait(lk, [&] { return stop || !pending.empty() || !updates.empty(); });
+            }
+        }
+    }
+};
+
+namespace {
+
+int64_t plan_bytes(const Core& x, const ceos_server_config& c, const ceos::QueryConfig& qc) {
+    int64_t b = (int64_t)c.max_batch * (x.d * 4 + 3 * 4 + (int64_t)c.k_max * 8);
+    b += (int64_t)c.max_batch * (x.DR() * 4 + (int64_t)qc.s_max * 4);
+    b += (int64_t)qc.grid * (qc.cand_stride * 16 + (int64_t)ceos::kSearchStashList * 16 + (4ll << qc.table_log2));
+    return b;
+}
+
+int make_server(Core& x, ceos_index* hi, ceos_stream* hs, const ceos_server_config* cfg, ceos_server** out) {
+    return guarded([&] {
+        if (!out || !cfg) throw ArgError("null argument");
+        if (cfg->max_batch <= 0 || cfg->k_max <= 0 || cfg->s_max <= 0 || cfg->c_max <= 0)
+            throw ArgError("max_batch, k_max, s_max and c_max must be positive");
+        if (!x.built) throw StateError("index is not built");
+        auto srv = std::make_unique<ceos_server>();
+        srv->core = &x;
+        srv->index = hi;
+        srv->stream = hs;
+        srv->cfg = *cfg;
+        int want = std::max(1, cfg->inflight);
+        ceos::QueryConfig qc;
+        {
+            std::lock_guard<std::mutex> g(x.mu);
+            qc = x.search_config_with(cfg->s_max, cfg->c_max, cfg->k_max, 0, 0);
+        }
+        const int64_t per = plan_bytes(x, *cfg, qc);
+        if (cfg->hbm_budget > 0) {
+            const int64_t fit = cfg->hbm_budget / per;
+            if (fit < 1)
+                throw MemError("hbm_budget of " + std::to_string(cfg->hbm_budget) + " bytes is below one batch slot (" +
+                               std::to_string(per) + " bytes)");
+            want = (int)std::min<int64_t>(want, fit);
+        }
+        for (int i = 0; i < want; ++i) {
+            auto sl = std::make_unique<Slot>();
+            const int rc = hi ? ceos_plan_create(hi, cfg->max_batch, cfg->k_max, cfg->s_max, cfg->c_max, 0, &sl->plan)
+                              : ceos_stream_plan_create(hs, cfg->max_batch, cfg->k_max, cfg->s_max, cfg->c_max, 0,
+                                                        &sl->plan);
+            if (rc != CEOS_OK) throw DevError(g_err);
+            HIP_CHECK(hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking));
+            HIP_CHECK(hipEventCreateWithFlags(&sl->ev, hipEventDisableTiming));
+            sl->hQ.alloc((size_t)cfg->max_batch * x.d);
+            sl->hk.alloc(cfg->max_batch);
+            sl->hs.alloc(cfg->max_batch);
+            sl->hc.alloc(cfg->max_batch);
+            sl->hI.alloc((size_t)cfg->max_batch * cfg->k_max);
+            sl->hP.alloc((size_t)cfg->max_batch * cfg->k_max);
+            srv->slots.push_back(std::move(sl));
+        }
+        srv->device_bytes = per * want;
+        ceos_server* p = srv.release();
+        p->worker = std::thread([p] {
+            try {
+                p->run();
+            } catch (const std::exception& e) {
+                std::lock_guard<std::mutex> g(p->mu);
+                p->fail_all(e.what());
+            }
+        });
+        *out = p;
+    });
+}
+
+void check_req(const ceos_server* s, int k, int sv, int c) {
+    if (k < 0 || k > s->cfg.k_max || sv <= 0 || sv > s->cfg.s_max || c <= 0 || c > s->cfg.c_max)
+        throw ArgError("k, n_probed_vectors or n_cand outside the server's bounds");
+}
+
+}  // namespace
+
+extern "C" {
+
+int ceos_server_create(ceos_index* h, const ceos_server_config* cfg, ceos_server** out) {
+    if (!h) {
+        g_err = "null handle";
+        return CEOS_EINVAL;
+    }
+    return make_server(h->core, h, nullptr, cfg, out);
+}
+
+int ceos_stream_server_create(ceos_stream* h, const ceos_server_config* cfg, ceos_server** out) {
+    if (!h) {
+        g_err = "null handle";
+        return CEOS_EINVAL;
+    }
+    return make_server(h->core, nullptr, h, cfg, out);
+}
+
+void ceos_server_destroy(ceos_server* s) { delete s; }
+
+int ceos_server_submit(ceos_server* s, const float* Q, int n, int d, const int32_t* k, const int32_t* sv,
+                       const int32_t* c, const double* arrival_s, int64_t* first_ticket) {
+    return guarded([&] {
+        if (!s || !Q || !k || !sv || !c || !first_ticket) throw ArgError("null argument");
+        if (d != s->core->d) throw ArgError("feature dimension does not match the index");
+        if (n <= 0) throw ArgError("n must be positive");
+        for (int i = 0; i < n; ++i) check_req(s, k[i], sv[i], c[i]);
+        const auto t0 = Clock::now();
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            if (!s->worker_error.empty()) throw DevError("server stopped: " + s->worker_error);
+            if (s->cfg.max_pending > 0 && (int64_t)s->pending.size() + n > s->cfg.max_pending)
+                throw MemError("server queue is full");
+            *first_ticket = s->next_ticket;
+            for (int i = 0; i < n; ++i) {
+                Request r;
+                r.q.assign(Q + (size_t)i * d, Q + (size_t)(i + 1) * d);
+                r.k = k[i];
+                r.s = sv[i];
+                r.c = c[i];
+                r.arrival = arrival_s ? t0 + std::chrono::duration_cast<Clock::duration>(
+                                                 std::chrono::duration<double>(arrival_s[i]))
+                                      : t0;
+                if (!s->any_arrival || r.arrival < s->first_arrival) s->first_arrival = r.arrival;
+                s->any_arrival = true;
+                const int64_t t = s->next_ticket++;
+                s->pending.push({r.arrival, t});
+                s->reqs.emplace(t, std::move(r));
+            }
+            s->submitted += n;
+        }
+        s->cv_work.notify_one();
+    });
+}
+
+int ceos_server_result(ceos_server* s, int64_t ticket, int wait, int32_t* ids, float* ips, int* k_out,
+                       double* latency_ms) {
+    if (!s) {
+        g_err = "null server";
+        return CEOS_EINVAL;
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) {
+        g_err = "unknown ticket " + std::to_string(ticket);
+        return CEOS_EINVAL;
+    }
+    if (!it->second.done) {
+        if (!wait) {
+            g_err = "not finished";
+            return CEOS_EAGAIN;
+        }
+        s->cv_done.wait(lk, [&] { return it->second.done; });
+    }
+    Request& r = it->second;
+    if (r.failed) {
+        g_err = s->worker_error;
+        s->reqs.erase(it);
+        return CEOS_EDEVICE;
+    }
+    if (ids) std::copy(r.ids.begin(), r.ids.end(), ids);
+    if (ips) std::copy(r.ips.begin(), r.ips.end(), ips);
+    if (k_out) *k_out = r.k;
+    if (latency_ms) *latency_ms = r.latency_ms;
+    s->reqs.erase(it);
+    g_err.clear();
+    return CEOS_OK;
+}
+
+int ceos_server_update(ceos_server* s, const float* Xnew, int64_t n_new, int d, int n_del) {
+    return guarded([&] {
+        if (!s) throw ArgError("null server");
+        if (!s->stream) throw ArgError("updates need a streamCEOs server");
+        if (n_new < 0 || n_del < 0 || (n_new > 0 && !Xnew)) throw ArgError("invalid update");
+        if (d != s->core->d) throw ArgError("feature dimension does not match the index");
+        Update u;
+        u.X.assign(Xnew, Xnew + (size_t)n_new * d);
+        u.n_new = n_new;
+        u.n_del = n_del;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->updates.push_back(std::move(u));
+        }
+        s->cv_work.notify_one();
+    });
+}
+
+int ceos_server_delete(ceos_server* s, const int32_t* ids, int64_t count) {
+    return guarded([&] {
+        if (!s) throw ArgError("null server");
+        if (!s->stream) throw ArgError("deletes need a streamCEOs server");
+        if (count < 0 || (count > 0 && !ids)) throw ArgError("invalid delete");
+        Update u;
+        u.is_delete = true;
+        u.ids.assign(ids, ids + count);
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->updates.push_back(std::move(u));
+        }
+        s->cv_work.notify_one();
+    });
+}
+
+int ceos_server_drain(ceos_server* s) {
+    return guarded([&] {
+        if (!s) throw ArgError("null server");
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.wait(lk, [&] {
+            if (!s->worker_error.empty()) return true;
+            if (!s->pending.empty() || !s->updates.empty()) return false;
+            for (auto& sl : s->slots)
+                if (sl->busy) return false;
+            return true;
+        });
+        if (!s->worker_error.empty()) throw DevError(s->worker_error);
+        if (!s->update_error.empty()) {
+            const std::string e = s->update_error;
+            s->update_error.clear();
+            throw ArgError(e);
+        }
+    });
+}
+
+int ceos_server_get_stats(ceos_server* s, ceos_server_stats* out) {
+    return guarded([&] {
+        if (!s || !out) throw ArgError("null argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        ceos_server_stats st{};
+        st.submitted = s->submitted;
+        st.completed = s->completed;
+        st.batches = s->batches;
+        st.mean_batch = s->batches ? (double)s->batched_queries / (double)s->batches : 0.0;
+        std::vector<double> v = s->lat_ms;
+        if (!v.empty()) {
+            std::sort(v.begin(), v.end());
+            auto pct = [&](double p) { return v[std::min(v.size() - 1, (size_t)(p * (double)(v.size() - 1) + 0.5))]; };
+            st.p50_ms = pct(0.50);
+            st.p99_ms = pct(0.99);
+            st.max_ms = v.back();
+            st.elapsed_s = std::chrono::duration<double>(s->last_done - s->first_arrival).count();
+            st.qps = st.elapsed_s > 0 ? (double)s->completed / st.elapsed_s : 0.0;
+        }
+        st.inflight_slots = (int)s->slots.size();
+        st.device_bytes = s->device_bytes;
+        *out = st;
+    });
+}
+
+int ceos_server_reset_stats(ceos_server* s) {
+    return guarded([&] {
+        if (!s) throw ArgError("null server");
+        std::lock_guard<std::mutex> g(s->mu);
+        s->submitted = s->completed = s->batches = s->batched_queries = 0;
+        s->lat_ms.clear();
+        s->any_arrival = false;
+    });
+}
+
+}  // extern "C"

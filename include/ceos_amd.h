@@ -32,6 +32,7 @@ extern "C" {
 #define CEOS_EDEVICE 2  /* HIP runtime / kernel failure */
 #define CEOS_ESTATE 3   /* index not built / wrong call order */
 #define CEOS_ENOMEM 4   /* device allocation failed or budget exceeded */
+#define CEOS_EAGAIN 5   /* not finished yet (ceos_server_result without wait) */
 
 typedef struct ceos_index ceos_index;
 typedef struct ceos_stream ceos_stream;
@@ -146,6 +147,56 @@ int ceos_plan_search(ceos_plan* p, const float* Q, int nq, int d, const int32_t*
                      const int32_t* c, int32_t* ids, float* ips);
 /* Launch shape in use (after tuning) and the tuned kernel time in ms (0 if untuned). */
 int ceos_plan_config(const ceos_plan* p, int* wg_per_cu, int* table_log2, double* tuned_ms);
+
+/* -------------------------------- serving --------------------------------
+ * No reference counterpart: continuous batching over a resident index. Requests (one
+ * query each, with its own k, n_probed_vectors and n_cand) are queued in arrival order;
+ * a worker thread takes every request that has arrived, up to max_batch, into the next
+ * free batch slot (a captured search plan with its own stream and pinned staging) as
+ * soon as one is free, so batches overlap on the device and requests that arrive while
+ * others run are admitted as slots free up. The number of slots is `inflight`, capped by
+ * hbm_budget (bytes of device memory for the slots' buffers and scratch; 0 = no cap).
+ * A request's results equal those of the same query searched alone. On a streamCEOs
+ * server, updates and deletes are applied between batches in submission order. */
+typedef struct ceos_server ceos_server;
+typedef struct {
+    int max_batch;       /* queries per batch slot */
+    int k_max, s_max, c_max;  /* bounds of the per-request parameters */
+    int inflight;        /* batch slots (>= 1), capped by hbm_budget */
+    int64_t hbm_budget;  /* device bytes the slots may hold; 0 = no cap */
+    int64_t max_pending; /* queued requests before submit fails with CEOS_ENOMEM; 0 = no cap */
+} ceos_server_config;
+typedef struct {
+    int64_t submitted, completed, batches;
+    double mean_batch;   /* queries per batch */
+    double p50_ms, p99_ms, max_ms;  /* arrival -> results ready */
+    double elapsed_s;    /* first arrival -> last completion */
+    double qps;          /* completed / elapsed_s */
+    int inflight_slots;
+    int64_t device_bytes;
+} ceos_server_stats;
+
+int ceos_server_create(ceos_index* h, const ceos_server_config* cfg, ceos_server** out);
+int ceos_stream_server_create(ceos_stream* h, const ceos_server_config* cfg, ceos_server** out);
+/* Waits for every queued request, then stops the worker. */
+void ceos_server_destroy(ceos_server* s);
+/* Queues n requests (rows of Q, copied). arrival_s (optional) gives each request's
+ * arrival time in seconds after this call (open-loop load); NULL = now. Tickets are
+ * first_ticket .. first_ticket + n - 1. */
+int ceos_server_submit(ceos_server* s, const float* Q, int n, int d, const int32_t* k, const int32_t* n_probed_vectors,
+                       const int32_t* n_cand, const double* arrival_s, int64_t* first_ticket);
+/* Results of one request: k entries into ids/ips (descending, zero padded), its k and
+ * its latency (arrival to results ready). wait = 0 returns CEOS_EAGAIN while it runs.
+ * A ticket can be collected once. */
+int ceos_server_result(ceos_server* s, int64_t ticket, int wait, int32_t* ids, float* ips, int* k_out,
+                       double* latency_ms);
+/* streamCEOs servers: queued like requests, applied between batches. */
+int ceos_server_update(ceos_server* s, const float* Xnew, int64_t n_new, int d, int n_del);
+int ceos_server_delete(ceos_server* s, const int32_t* ids, int64_t count);
+/* Waits until nothing is queued or running; reports a failed update. */
+int ceos_server_drain(ceos_server* s);
+int ceos_server_get_stats(ceos_server* s, ceos_server_stats* out);
+int ceos_server_reset_stats(ceos_server* s);
 
 /* --------------------------- class streamCEOs --------------------------- */
 
