@@ -154,7 +154,7 @@ struct ScratchSet {
         cand.ensure((size_t)cfg.grid * cfg.cand_stride);
         pairs.ensure((size_t)cfg.grid * cfg.cand_stride);
         stash.ensure((size_t)cfg.grid * ceos::kSearchStashList);
-        tkeys.ensure((size_t)cfg.grid << cfg.table_log2);
+        tkeys.ensure((size_t)cfg.grid * ceos::search_tile_slots(cfg.table_log2));
         qproj.ensure((size_t)std::max(nq, 1) * DR);
         probes.ensure((size_t)std::max(nq, 1) * cfg.s_max);
         if (!work.p) work.alloc(1);

@@ -100,7 +100,7 @@ struct QueryScratch {
     uint64_t* cand;    // grid * cand_stride candidate keys
     uint2* pairs;      // grid * cand_stride (slot, value) per probed bucket position
     uint4* stash;      // grid * kSearchStashList entries that missed both tile buckets
-    uint32_t* keys;    // grid << table_log2 tile ids during the ordered pass
+    uint32_t* keys;    // grid * search_tile_slots(table_log2) tile ids during the ordered pass
     int32_t* work;     // work counter (1 int)
     int32_t* status;   // error flags (1 int): bit0 = hash table overflow
 };
@@ -112,6 +112,12 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
                                 const QueryConfig& cfg, const QueryScratch& scr, hipStream_t st);
 
 int search_lds_bytes(int d, int table_log2, int s_max, int c_max, int k_max);
+// Score-tile slots: 2^table_log2 hashed slots plus a linear-probing overflow region.
+constexpr int kSearchStashSlots = 2048;
+__host__ __device__ inline int search_tile_slots(int table_log2) {
+    const int t = 1 << table_log2;
+    return t + (t / 16 < kSearchStashSlots ? t / 16 : kSearchStashSlots);
+}
 int search_ept(int mp);
 constexpr int kSearchThreads = 512;
 constexpr int kMaxProbedPoints = 4096;

@@ -7,7 +7,8 @@
 //    list ids 2c (close, p >= 0) / 2c+1 (far, p < 0) (src/CEOs.cpp:517-548, 562-564).
 // 3. k_search_fused (persistent, one 512-thread workgroup per query at a time, next
 //    query from a global work counter so uneven k / s / c balance across the chip):
-//    a. score accumulation in two phases over an LDS tile of 2^table_log2 slots.
+//    a. score accumulation in two phases over an LDS tile of 2^table_log2 hashed slots
+//       plus a small overflow region.
 //       Phase 1 gives every distinct id of the probed lists a slot (two-choice
 //       buckets of four 4-byte ids, claims by atomicMin between barriers, a small
 //       linear-probing stash for the rare leftovers) and records each bucket
@@ -42,18 +43,22 @@ constexpr int kBufEntries = 8;         // bucket entries per thread per chunk
 constexpr uint32_t kUnset = 0xffffffffu;       // sum bits of a slot before its first (ordered) add
 constexpr uint32_t kStashMark = 0xfffffffeu;   // phase-1 slot value: entry sent to the stash
 constexpr uint32_t kToPlace = 0xfffffffdu;     // phase-1 slot value: entry not placed yet
-constexpr int kStashSlots = 2048;              // linear-probing stash at the end of the tile
 constexpr int kEmitU = 8;                      // tile slots per thread per emission step
-constexpr int kSelBins = 4096;                 // candidate select: value histogram bins
-constexpr int kBoundMax = 1024;                // boundary-bin keys selected in LDS (else in scratch)
+constexpr int kSelBins = 2048;                 // candidate select: value histogram bins
+constexpr int kBoundMax = 512;                 // boundary-bin keys selected in LDS (else in scratch)
 constexpr int kStashList = kSearchStashList;   // pending stash entries per pass (global list)
 
-__device__ __forceinline__ uint32_t bucket1(uint32_t id, uint32_t nbk) {
-    return (uint32_t)(((uint64_t)(id * 0x9e3779b1u) * nbk) >> 32);
+// One multiplicative hash per id (a single 32-bit multiply) serves both bucket choices
+// (its top bits, then the next bits; 2^lognbk buckets), the claim rotation (bits 4-5)
+// and the pass partition (high half folded onto the low half).
+__device__ __forceinline__ uint32_t idhash(uint32_t id) { return id * 0x9e3779b1u; }
+__device__ __forceinline__ uint32_t bucket1(uint32_t h, int lognbk) { return h >> (32 - lognbk); }
+__device__ __forceinline__ uint32_t bucket2(uint32_t h, uint32_t b1, int lognbk) {
+    const uint32_t b2 = (h >> (32 - 2 * lognbk)) & ((1u << lognbk) - 1u);
+    return b2 == b1 ? b1 ^ 1u : b2;
 }
-__device__ __forceinline__ uint32_t bucket2(uint32_t id, uint32_t b1, uint32_t nbk) {
-    const uint32_t b2 = (uint32_t)(((uint64_t)((id ^ 0x5bd1e995u) * 0x85ebca6bu) * nbk) >> 32);
-    return b2 == b1 ? (b1 + 1 == nbk ? 0u : b1 + 1) : b2;
+__device__ __forceinline__ uint32_t pass_of(uint32_t h, uint32_t P) {
+    return (((h ^ (h >> 15)) & 0xffffu) * P) >> 16;
 }
 
 template <typename K>
@@ -81,7 +86,7 @@ __host__ __device__ inline SLayout s_layout(int d, int tlog2, int s_max, int c_m
     o.hist_off = off; off += kSelBins * 4;
     o.u_off = off;
     o.tab_off = off;
-    const int endA = off + (4 << tlog2);
+    const int endA = off + 4 * search_tile_slots(tlog2);
     o.rk_off = off;
     o.tk_off = off + al16(c_max * 8);
     const int endB = o.tk_off + next_pow2(k_max) * 8;
@@ -151,12 +156,13 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
     uint64_t* tile64 = reinterpret_cast<uint64_t*>(lds + lo.tab_off);
     uint64_t* rk = reinterpret_cast<uint64_t*>(lds + lo.rk_off);
     uint64_t* tk = reinterpret_cast<uint64_t*>(lds + lo.tk_off);
-    const int T = 1 << cfg.table_log2;
-    const int sslots = min(kStashSlots, T / 8);          // stash keys at the end of the tile
-    const uint32_t nbk = (uint32_t)((T - sslots) / 4);  // main buckets of 4 keys (16 bytes)
+    const int Tmain = 1 << cfg.table_log2;         // hashed slots: 2^lognbk buckets of 4 keys (16 bytes)
+    const int lognbk = cfg.table_log2 - 2;
+    const int T = search_tile_slots(cfg.table_log2);  // plus the overflow (stash) region at the end
+    const int sslots = T - Tmain;
     uint64_t* cand = scr.cand + (int64_t)blockIdx.x * cfg.cand_stride;
     uint2* pairs = scr.pairs + (int64_t)blockIdx.x * cfg.cand_stride;
-    uint32_t* keys_g = scr.keys + ((int64_t)blockIdx.x << cfg.table_log2);
+    uint32_t* keys_g = scr.keys + (int64_t)blockIdx.x * search_tile_slots(cfg.table_log2);
     uint4* stash_list = scr.stash + (int64_t)blockIdx.x * kStashList;
     const int tid = threadIdx.x, wave = tid / kWave, ln = tid % kWave;
     float frac = 0.5f;  // running estimate of distinct ids / entries for this workgroup
@@ -226,18 +232,18 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         // claim the same slot and win or lose together. All reads of a chunk are issued
         // before any of them is used.
         auto phase1_chunk = [&](const Entry (&buf)[CH][EPT], int j0, uint32_t pass) {
-            uint32_t id[B], slot[B], b1[B], b2[B];
+            uint32_t id[B], slot[B], b1[B], b2[B], hh[B];
             float v[B];
 #pragma unroll
             for (int b = 0; b < B; ++b) {
                 const int j = j0 + b / EPT, t = tid + (b % EPT) * kSNT;
                 id[b] = buf[b / EPT][b % EPT].x - rows.base;
                 v[b] = __uint_as_float(buf[b / EPT][b % EPT].y);
-                const bool valid = j < sq && t < lims[min(j, jlast)] &&
-                                   (P == 1 || (uint32_t)(((uint64_t)(id[b] * 0x85ebca77u) * P) >> 32) == pass);
+                hh[b] = idhash(id[b]);
+                const bool valid = j < sq && t < lims[min(j, jlast)] && (P == 1 || pass_of(hh[b], P) == pass);
                 slot[b] = valid ? kToPlace : kEmptyKey;  // kEmptyKey: not in this pass
-                b1[b] = bucket1(id[b], nbk);
-                b2[b] = bucket2(id[b], b1[b], nbk);
+                b1[b] = bucket1(hh[b], lognbk);
+                b2[b] = bucket2(hh[b], b1[b], lognbk);
             }
             uint4 k1[B], k2[B];
 #pragma unroll
@@ -262,7 +268,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 const uint32_t hm = in1 ? m1 : m2;
                 // a new id takes the first empty slot at or after a per-id start position
                 // (cyclically), so that different ids rarely claim the same slot
-                const uint32_t r = (x * 0x2c1b3c6du) >> 30;
+                const uint32_t r = (hh[b] >> 4) & 3u;
                 const uint32_t em = in1 ? e1 : e2;
                 const uint32_t rot = ((em | (em << 4)) >> r) & 15u;
                 const uint32_t within = hm != 0u ? (uint32_t)__builtin_ctz(hm | 16u)
@@ -307,7 +313,8 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             for (int i = tid; i < ns; i += kSNT) {
                 const uint4 e = stash_list[i];
                 const uint32_t idv = e.x;
-                const uint32_t b1 = bucket1(idv, nbk), b2 = bucket2(idv, b1, nbk);
+                const uint32_t hv = idhash(idv);
+                const uint32_t b1 = bucket1(hv, lognbk), b2 = bucket2(hv, b1, lognbk);
                 const uint4 p = reinterpret_cast<const uint4*>(keys)[b1];
                 const uint4 q2 = reinterpret_cast<const uint4*>(keys)[b2];
                 const uint32_t m1 = (p.x == idv ? 1u : 0u) | (p.y == idv ? 2u : 0u) | (p.z == idv ? 4u : 0u) |
@@ -317,7 +324,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 uint32_t found = m1 ? 4u * b1 + (uint32_t)__builtin_ctz(m1)
                                     : m2 ? 4u * b2 + (uint32_t)__builtin_ctz(m2) : kEmptyKey;
                 if (found == kEmptyKey) {
-                    uint32_t h = (idv * 0x9e3779b1u) & (uint32_t)(sslots - 1);
+                    uint32_t h = (hv ^ (hv >> 13)) & (uint32_t)(sslots - 1);
                     for (int probe = 0; probe < sslots; ++probe) {
                         const uint32_t si = (uint32_t)(T - sslots) + h;
                         const uint32_t k = keys[si];
