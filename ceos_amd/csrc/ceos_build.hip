@@ -5,7 +5,7 @@
 // (src/CEOs.cpp:335-372):
 //   1. project a strided sample of S points, column-major [DR][S];
 //   2. per list (close/far of each column) take the sample value at a rank chosen so
-//      that about 2*top_m of all n points pass it;
+//      that about 2.75*top_m of all n points pass it;
 //   3. project all n points once more and append (id, value) of every point that passes
 //      a list's threshold into that list's candidate segment;
 //   4. per list, exact top-m of the candidates by (value desc, id asc), sorted.
@@ -97,6 +97,32 @@ hipError_t launch_project_rows(const IndexDims& dm, const float* X, int64_t firs
 }
 
 // ---------------------------------------------------------------------------------
+// out[c * rows + r] = in[r * cols + c], through 64 x 64 LDS tiles (coalesced both ways).
+__global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in, int64_t rows, int64_t cols,
+                                                   float* __restrict__ out) {
+    __shared__ float tile[64][65];
+    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    for (int i = ty; i < 64; i += 4) {
+        const int64_t r = r0 + i, c = c0 + tx;
+        if (r < rows && c < cols) tile[i][tx] = in[r * cols + c];
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        const int64_t c = c0 + i, r = r0 + tx;
+        if (r < rows && c < cols) out[c * rows + r] = tile[tx][i];
+    }
+}
+
+hipError_t launch_transpose(const float* in, int64_t rows, int64_t cols, float* out, hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    if ((rows + 63) / 64 > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_transpose, dim3((unsigned)((cols + 63) / 64), (unsigned)((rows + 63) / 64)), dim3(256), 0,
+                       st, in, rows, cols, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBuildNT) void k_sample_thresholds(const float* __restrict__ sample, int64_t S,
                                                                 int rank, float* __restrict__ thr) {
     __shared__ SelectShared sh;
@@ -121,6 +147,12 @@ hipError_t launch_sample_thresholds(const float* sample, int64_t S, int DR, int 
 }
 
 // ---------------------------------------------------------------------------------
+// Hits (list, point, value) are staged per wave in LDS, placed by ballot + lane prefix
+// (no atomics), and flushed when the stage may overflow: every lane then takes staged
+// hits and reserves their positions with global atomics issued back to back, so one
+// round trip serves up to kStage hits instead of one divergent round trip per hit.
+constexpr int kStage = 512;  // staged hits per wave
+
 template <int L>
 __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const float* __restrict__ X, int64_t n,
                                                           const uint32_t* __restrict__ signs,
@@ -133,6 +165,10 @@ __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const fl
     extern __shared__ float lds_all[];
     const int wave = threadIdx.x / kWave, ln = threadIdx.x % kWave;
     float* lds = lds_all + wave * P::kLdsWords;
+    // per-wave stage after the FWHT tiles: list, point id, value bits
+    uint32_t* st_list = reinterpret_cast<uint32_t*>(lds_all + kBuildWaves * P::kLdsWords) + wave * 3 * kStage;
+    uint32_t* st_id = st_list + kStage;
+    uint32_t* st_val = st_id + kStage;
     const int r = blockIdx.y;
     const int W = sign_words(L);
     const uint32_t* sg = signs + (size_t)r * 3 * W;
@@ -151,6 +187,28 @@ __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const fl
     }
     if (!__any((act_c | act_f) != 0ull)) return;
 
+    uint32_t staged = 0;  // wave-uniform
+    auto flush = [&]() {
+        wave_sync();
+        constexpr int U = kStage / kWave;
+        int32_t pos[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {  // reserve: atomics in flight together
+            const uint32_t i = (uint32_t)(u * kWave + ln);
+            pos[u] = i < staged ? atomicAdd(&count[st_list[i]], 1) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = (uint32_t)(u * kWave + ln);
+            if (i < staged) {
+                const uint32_t list = st_list[i];
+                if (pos[u] < cap[list]) ent[base[list] + pos[u]] = make_uint2(st_id[i], st_val[i]);
+            }
+        }
+        wave_sync();
+        staged = 0;
+    };
+
     for (int64_t p = (int64_t)blockIdx.x * kBuildWaves + wave; p < n; p += (int64_t)gridDim.x * kBuildWaves) {
         float xr[P::XE];
         P::load_x(X + p * dm.d, dm.d, xr, ln);
@@ -159,19 +217,30 @@ __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const fl
 #pragma unroll
         for (int reg = 0; reg < P::OE; ++reg) {
             const float v = y[reg];
-            const int c = r * dm.D + reg * kWave + ln;
-            if (((act_c >> reg) & 1ull) && v >= th_c[reg]) {
-                const int list = 2 * c;
-                const int pos = atomicAdd(&count[list], 1);
-                if (pos < cap[list]) ent[base[list] + pos] = make_uint2((uint32_t)p, __float_as_uint(v));
-            }
-            if (((act_f >> reg) & 1ull) && -v >= th_f[reg]) {
-                const int list = 2 * c + 1;
-                const int pos = atomicAdd(&count[list], 1);
-                if (pos < cap[list]) ent[base[list] + pos] = make_uint2((uint32_t)p, __float_as_uint(-v));
+            const uint32_t c = (uint32_t)(r * dm.D + reg * kWave + ln);
+            const bool hc = ((act_c >> reg) & 1ull) && v >= th_c[reg];
+            const bool hf = ((act_f >> reg) & 1ull) && -v >= th_f[reg];
+            const uint64_t mc = __ballot(hc), mf = __ballot(hf);
+            if (mc | mf) {  // wave-uniform
+                if (staged + (uint32_t)(__popcll(mc) + __popcll(mf)) > (uint32_t)kStage) flush();  // <= 128 fit after
+                if (hc) {
+                    const uint32_t i = staged + lanes_below(mc);
+                    st_list[i] = 2 * c;
+                    st_id[i] = (uint32_t)p;
+                    st_val[i] = __float_as_uint(v);
+                }
+                staged += (uint32_t)__popcll(mc);
+                if (hf) {
+                    const uint32_t i = staged + lanes_below(mf);
+                    st_list[i] = 2 * c + 1;
+                    st_id[i] = (uint32_t)p;
+                    st_val[i] = __float_as_uint(-v);
+                }
+                staged += (uint32_t)__popcll(mf);
             }
         }
     }
+    if (staged > 0) flush();
 }
 
 hipError_t launch_scan_append(const IndexDims& dm, const float* X, int64_t n, const uint32_t* signs,
@@ -180,7 +249,8 @@ hipError_t launch_scan_append(const IndexDims& dm, const float* X, int64_t n, co
     if (n <= 0) return hipSuccess;
     int gx = (int)std::min<int64_t>((n + kBuildWaves - 1) / kBuildWaves, std::max(1, 2048 / dm.R) * 4);
     CEOS_SWITCH_L(dm.L, {
-        const size_t lds = (size_t)kBuildWaves * Proj<LL>::kLdsWords * sizeof(float);
+        const size_t lds = (size_t)kBuildWaves * (Proj<LL>::kLdsWords * sizeof(float) + 3 * kStage * 4);
+        allow_lds(k_scan_append<LL>, lds);
         hipLaunchKernelGGL(k_scan_append<LL>, dim3(gx, dm.R), dim3(kBuildNT), lds, st, dm, X, n, signs, thr,
                            active, base, cap, count, ent);
     });

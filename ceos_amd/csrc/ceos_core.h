@@ -290,18 +290,23 @@ struct Core {
         const IndexDims dm = dims();
         if (m > n) throw ArgError("top_m must not exceed the number of points");
 
-        int64_t S_target = std::max<int64_t>(65536, (64 * n + m - 1) / m);
-        S_target = std::min<int64_t>(S_target, (int64_t(4) << 30) / (4LL * DR_));
+        // sample size: about 32 n/m points (at least 32768); the threshold is the sample
+        // value of rank 2.5 m S/n + 8 (about 90), so about 2.75 m points pass it and a list
+        // under m or over its 4m + 256 cap (both checked; rescanned) is a > 5 sigma event
+        int64_t S_target = std::max<int64_t>(32768, (32 * n + m - 1) / m);
+        S_target = std::min<int64_t>(S_target, std::min<int64_t>((int64_t(2) << 30) / (4LL * DR_), 65535LL * 64));
         S_target = std::max<int64_t>(S_target, 1);
         const int64_t stride = std::max<int64_t>(1, n / S_target);
         const int64_t S = (n + stride - 1) / stride;
         const bool exact = stride == 1;
-        const int rank = exact ? m : (int)std::min<int64_t>(S, (2 * (int64_t)m * S + n - 1) / n + 8);
+        const int rank = exact ? m : (int)std::min<int64_t>(S, (5 * (int64_t)m * S + 2 * n - 1) / (2 * n) + 8);
 
         DBuf<float> thr(nl);
         {
-            DBuf<float> sample((size_t)DR_ * S);
-            HIP_CHECK(ceos::launch_project_rows(dm, X.p, 0, stride, S, signs.p, sample.p, S, true, s_));
+            // project the sample row-major (coalesced), then transpose to one column per list
+            DBuf<float> rowsP((size_t)DR_ * S), sample((size_t)DR_ * S);
+            HIP_CHECK(ceos::launch_project_rows(dm, X.p, 0, stride, S, signs.p, rowsP.p, DR_, false, s_));
+            HIP_CHECK(ceos::launch_transpose(rowsP.p, S, DR_, sample.p, s_));
             HIP_CHECK(ceos::launch_sample_thresholds(sample.p, S, DR_, rank, thr.p, s_));
             HIP_CHECK(hipStreamSynchronize(s_));
         }
