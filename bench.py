@@ -54,7 +54,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="mixture_1m")
+    # default: the 1,000,000 x 960 uneven batch, the configuration the headline target
+    # (>= 20x the plain per-query path at >= 60% of the HBM roofline) is stated on
+    ap.add_argument("--workload", default="gist_like")
     ap.add_argument("--n", type=int, default=0, help="override number of points (testing only)")
     ap.add_argument("--nq", type=int, default=0, help="override number of queries (testing only)")
     ap.add_argument("--plain-queries", type=int, default=300)
