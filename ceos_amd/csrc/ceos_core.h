@@ -363,6 +363,8 @@ struct Core {
     }
     // The same for an explicit launch shape (0 = automatic), without kernel timing.
     ceos::QueryConfig search_config_with(int s_max, int c_max, int k_max, int wg_req, int tl_req) {
+        // a query never has more distinct candidates than its probed entries
+        c_max = (int)std::min<int64_t>(c_max, (int64_t)s_max * std::max(1, std::min(mp, m)));
         const int wg = wg_req > 0 ? wg_req : 1;
         const int budget = std::min(kMaxLdsBytes, (160 * 1024) / wg - 2048);
         int t = 0;
@@ -416,21 +418,25 @@ struct Core {
     }
 
     // Uniform-parameter coCEOs search on device pointers.
-    void search_device(const float* dQ, int nq, int k, int s_, int c_, int32_t* ids, float* ips, hipStream_t s__) {
+    // Uniform-parameter search on device pointers; hash = CEOs-Hash (every distinct
+    // candidate reranked, c unused).
+    void search_device(const float* dQ, int nq, int k, int s_, int c_, int32_t* ids, float* ips, hipStream_t s__,
+                       bool hash = false) {
         if (nq <= 0 || k == 0) return;
+        if (hash) c_ = 1;
         const ceos::QueryConfig cfg = search_config(s_, c_, k);
         const ceos::QueryScratch scr = scratch(cfg, nq);
-        ceos::QueryBatch qb{dQ, nq, nullptr, nullptr, nullptr, k, s_, c_, mp, nullptr, ids, ips};
+        ceos::QueryBatch qb{dQ, nq, nullptr, nullptr, nullptr, k, s_, c_, mp, nullptr, ids, ips, nullptr, hash ? 1 : 0};
         HIP_CHECK(ceos::launch_coceos_search(dims(), signs.p, bkt.p, bsize.p, rows(), qb, cfg, scr, s__));
     }
 
-    void search_host(const float* Q, int nq, int k, int s_, int c_, int32_t* ids, float* ips) {
+    void search_host(const float* Q, int nq, int k, int s_, int c_, int32_t* ids, float* ips, bool hash = false) {
         if (nq <= 0 || k == 0) return;
         DBuf<float> dQ((size_t)nq * d);
         DBuf<int32_t> dI((size_t)nq * k);
         DBuf<float> dP((size_t)nq * k);
         HIP_CHECK(hipMemcpyAsync(dQ.p, Q, (size_t)nq * d * 4, hipMemcpyHostToDevice, st()));
-        search_device(dQ.p, nq, k, s_, c_, dI.p, dP.p, st());
+        search_device(dQ.p, nq, k, s_, c_, dI.p, dP.p, st(), hash);
         HIP_CHECK(hipMemcpyAsync(ids, dI.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, st()));
         HIP_CHECK(hipMemcpyAsync(ips, dP.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, st()));
         HIP_CHECK(hipStreamSynchronize(st()));

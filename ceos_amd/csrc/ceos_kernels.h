@@ -84,6 +84,8 @@ struct QueryBatch {
     // optional: the batch size read on the device (captured plans launch for a maximum
     // batch nq and take the actual count from here, nq_dev[0] <= nq)
     const int32_t* nq_dev;
+    // CEOs-Hash search: rerank every distinct id of the probed lists, no score (c unused)
+    int hash;
 };
 
 struct QueryConfig {

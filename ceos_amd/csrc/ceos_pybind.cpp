@@ -4,8 +4,7 @@
 // Same class, method, argument and field names and defaults. Differences, all
 // stricter: invalid parameters raise ValueError instead of exit(1)
 // (src/CEOs.cpp:452-461); device failures raise RuntimeError; the GIL is released
-// while the device works. build_CEOs_Hash / search_CEOs_Hash / hash_search are not
-// provided (CEOs-Hash is outside this library's scope) and raise NotImplementedError.
+// while the device works.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 
@@ -33,11 +32,6 @@ void check(int rc) {
 
 void require_2d(const FArr& a, const char* what) {
     if (a.ndim() != 2) throw py::value_error(std::string(what) + " must be a 2-D array (rows x n_features)");
-}
-
-[[noreturn]] void not_provided(const char* what) {
-    PyErr_SetString(PyExc_NotImplementedError, what);
-    throw py::error_already_set();
 }
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -145,6 +139,41 @@ class PyCEOs {
         check(rc);
         std::printf("Constructing time (in seconds): %.3f\n", ms_since(t0) / 1000.0);
         std::fflush(stdout);
+    }
+
+    void build_CEOs_Hash(FArr X) {
+        require_2d(X, "dataset");
+        print_params("Building CEOs-Hash index...");
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_build_hash(h_, X.data(), X.shape(0), (int)X.shape(1));
+        }
+        check(rc);
+        std::printf("Constructing time (in seconds): %.3f\n", ms_since(t0) / 1000.0);
+        std::fflush(stdout);
+    }
+
+    py::tuple search_CEOs_Hash(FArr Q, int k, bool verbose) {
+        require_2d(Q, "queries");
+        const int nq = (int)Q.shape(0);
+        if (k < 0) throw py::value_error("n_neighbors must be >= 0");
+        py::array_t<int32_t> ids({nq, k});
+        py::array_t<float> ips({nq, k});
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_search_hash(h_, Q.data(), nq, (int)Q.shape(1), k, ids.mutable_data(), ips.mutable_data());
+        }
+        check(rc);
+        if (verbose) {
+            std::printf("n_queries: %d\nn_probedVectors: %d\nn_probedPoints: %d\nQuerying time: %.3f ms\n", nq, get_s(),
+                        get_mp(), ms_since(t0));
+            std::fflush(stdout);
+        }
+        return result(nq, k, ids, ips);
     }
 
     py::tuple search_coCEOs_Est(FArr Q, int k, bool verbose) {
@@ -356,6 +385,28 @@ class PyStream {
         }
         return py::make_tuple(ids, ips);
     }
+
+    py::tuple hash_search(FArr Q, int k, bool verbose) {
+        require_2d(Q, "queries");
+        const int nq = (int)Q.shape(0);
+        if (k < 0) throw py::value_error("n_neighbors must be >= 0");
+        py::array_t<int32_t> ids({nq, k});
+        py::array_t<float> ips({nq, k});
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_stream_hash_search(h_, Q.data(), nq, (int)Q.shape(1), k, ids.mutable_data(),
+                                             ips.mutable_data());
+        }
+        check(rc);
+        if (verbose) {
+            std::printf("n_queries: %d\nn_probedVectors: %d\nn_cand: %d\nQuerying time: %.3f ms\n", nq, get_s(), get_c(),
+                        ms_since(t0));
+            std::fflush(stdout);
+        }
+        return py::make_tuple(ids, ips);
+    }
     int64_t size() const { int64_t n; check(ceos_stream_size(h_, &n)); return n; }
     py::tuple get_buckets(int DR, int m) const {
         py::array_t<int32_t> cid({DR, m}), fid({DR, m}), cs(DR), fs(DR);
@@ -505,11 +556,9 @@ PYBIND11_MODULE(CEOs, m) {
         .def("build_coCEOs_Est", &PyCEOs::build_coCEOs_Est, py::arg("dataset"))
         .def("search_coCEOs_Est", &PyCEOs::search_coCEOs_Est, py::arg("queries"), py::arg("n_neighbors"),
              py::arg("verbose") = false)
-        .def("build_CEOs_Hash", [](PyCEOs&, py::object) { not_provided("CEOs-Hash is not provided by this library"); },
-             py::arg("dataset"))
-        .def("search_CEOs_Hash",
-             [](PyCEOs&, py::object, int, bool) -> py::tuple { not_provided("CEOs-Hash is not provided by this library"); },
-             py::arg("queries"), py::arg("n_neighbors"), py::arg("verbose") = false)
+        .def("build_CEOs_Hash", &PyCEOs::build_CEOs_Hash, py::arg("dataset"))
+        .def("search_CEOs_Hash", &PyCEOs::search_CEOs_Hash, py::arg("queries"), py::arg("n_neighbors"),
+             py::arg("verbose") = false)
         .def("make_plan", &PyCEOs::make_plan, py::arg("nq_max"), py::arg("k_max"), py::arg("s_max"),
              py::arg("c_max"), py::arg("autotune") = true, py::keep_alive<0, 1>())
         .def("search_batched", &PyCEOs::search_batched, py::arg("queries"), py::arg("n_neighbors"),
@@ -569,9 +618,8 @@ PYBIND11_MODULE(CEOs, m) {
         .def("update", &PyStream::update, py::arg("new_dataset"), py::arg("n_delPoints") = 0)
         .def("estimate_search", &PyStream::estimate_search, py::arg("queries"), py::arg("n_neighbors"),
              py::arg("verbose") = false)
-        .def("hash_search",
-             [](PyStream&, py::object, int, bool) -> py::tuple { not_provided("streamCEOs.hash_search is not provided by this library"); },
-             py::arg("queries"), py::arg("n_neighbors"), py::arg("verbose") = false)
+        .def("hash_search", &PyStream::hash_search, py::arg("queries"), py::arg("n_neighbors"),
+             py::arg("verbose") = false)
         .def("delete_ids", &PyStream::delete_ids, py::arg("ids"))
         .def("size", &PyStream::size)
         .def("get_buckets", &PyStream::get_buckets, py::arg("n_lists"), py::arg("top_m"));

@@ -109,6 +109,43 @@ int ceos_build_coceos_device(ceos_index* h, const float* dX, int64_t n, int d, v
     });
 }
 
+// build_CEOs_Hash: the same projections and per-column top-m lists as coCEOs (the
+// reference keeps only their ids, src/CEOs.cpp:709-902). With centering the stored
+// points are X minus its column mean; the reference's expression (CEOs.cpp:721-728,
+// a row-wise mean subtracted from every row) only type-checks when n equals d.
+int ceos_build_hash(ceos_index* h, const float* X, int64_t n, int d) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        check_matrix(X, n, d, c.d);
+        if (!c.centering) {
+            build_coceos_impl(h, X, n, d, false, c.st());
+            return;
+        }
+        std::vector<double> mean(d, 0.0);
+        for (int64_t i = 0; i < n; ++i)
+            for (int j = 0; j < d; ++j) mean[j] += X[i * d + j];
+        std::vector<float> Xc((size_t)n * d);
+        for (int64_t i = 0; i < n; ++i)
+            for (int j = 0; j < d; ++j) Xc[i * d + j] = X[i * d + j] - (float)(mean[j] / (double)n);
+        build_coceos_impl(h, Xc.data(), n, d, false, c.st());
+    });
+}
+
+// search_CEOs_Hash(queries, n_neighbors): the union of the first n_probed_points ids of
+// the n_probed_vectors lists, every candidate reranked (src/CEOs.cpp:904-1066).
+int ceos_search_hash(ceos_index* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        check_matrix(Q, nq, d, c.d);
+        c.validate_query(c.s, c.mp, 1, k, c.DR(), "n_proj * n_repeats");  // src/CEOs.cpp:906-915
+        c.search_host(Q, nq, k, c.s, 1, ids, ips, true);
+    });
+}
+
 int ceos_search_coceos(ceos_index* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips) {
     return guarded([&] {
         if (!h) throw ArgError("null handle");
@@ -524,6 +561,18 @@ int ceos_stream_estimate_search(ceos_stream* h, const float* Q, int nq, int d, i
         check_matrix(Q, nq, d, c.d);
         c.validate_query(c.s, c.mp, c.c, k, c.D, "number of projections");  // src/streamCEOs.cpp:386
         c.search_host(Q, nq, k, c.s, c.c, ids, ips);
+    });
+}
+
+// hash_search(queries, n_neighbors) — src/streamCEOs.cpp:621-877
+int ceos_stream_hash_search(ceos_stream* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        Core& c = h->core;
+        check_matrix(Q, nq, d, c.d);
+        c.validate_query(c.s, c.mp, 1, k, c.D, "number of projections");  // src/streamCEOs.cpp:623-633
+        c.search_host(Q, nq, k, c.s, 1, ids, ips, true);
     });
 }
 

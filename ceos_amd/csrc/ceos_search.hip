@@ -167,6 +167,10 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
     const int tid = threadIdx.x, wave = tid / kWave, ln = tid % kWave;
     float frac = 0.5f;  // running estimate of distinct ids / entries for this workgroup
     const int nq_run = qb.nq_dev ? min(qb.nq, qb.nq_dev[0]) : qb.nq;
+    // CEOs-Hash: candidates are all distinct ids (no scores); their ids go to idlist
+    // (the `pairs` scratch, unused in this mode) and their rerank keys to `cand`
+    const bool hash_mode = qb.hash != 0;
+    uint32_t* idlist = reinterpret_cast<uint32_t*>(pairs);
 
     for (;;) {
         if (tid == 0) misc[0] = atomicAdd(scr.work, 1);
@@ -301,7 +305,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     else
                         misc[4] = 1;  // stash list full: the pass restarts with more partitions
                 }
-                if (j < sq && t < lims[min(j, jlast)])
+                if (!hash_mode && j < sq && t < lims[min(j, jlast)])
                     pairs[(int64_t)j * mp + t] = make_uint2(slot[b] == kStashMark ? kEmptyKey : slot[b],
                                                             __float_as_uint(v[b]));
             }
@@ -338,7 +342,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     }
                     if (found == kEmptyKey) misc[4] = 1;  // stash full
                 }
-                if (found != kEmptyKey) pairs[e.z].x = found;
+                if (found != kEmptyKey && !hash_mode) pairs[e.z].x = found;
             }
         };
         uint32_t kmin = 0xffffffffu, kmax = 0u;  // fkey range of every sum written in phase 2
@@ -402,6 +406,23 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     __syncthreads();
                 }
                 if (misc[4] != 0) { overflow = true; break; }
+                if (hash_mode) {  // every id of this pass is a candidate (src/CEOs.cpp:1005-1026)
+                    for (int i0 = 0; i0 < T; i0 += kEmitU * kSNT) {  // uniform trip count (wave-level append)
+                        uint32_t kv[kEmitU], cnt = 0;
+#pragma unroll
+                        for (int u = 0; u < kEmitU; ++u) {
+                            const int i = i0 + u * kSNT + tid;
+                            kv[u] = i < T ? keys[i] : kEmptyKey;
+                            cnt += kv[u] != kEmptyKey ? 1u : 0u;
+                        }
+                        uint32_t pos = wave_append_n<4>(cnt, reinterpret_cast<uint32_t*>(&misc[2]));
+#pragma unroll
+                        for (int u = 0; u < kEmitU; ++u)
+                            if (kv[u] != kEmptyKey) idlist[pos++] = kv[u];
+                    }
+                    __syncthreads();
+                    continue;
+                }
                 // ids to scratch; the tile becomes the sums ("unset" until the first value)
                 {
                     uint32_t occ = 0;
@@ -461,7 +482,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 __syncthreads();
             }
             if (!overflow) {
-                ncand = P == 1 ? misc[6] : misc[2];
+                ncand = (P == 1 && !hash_mode) ? misc[6] : misc[2];
                 break;
             }
             __syncthreads();
@@ -475,8 +496,10 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         if (E > 0) frac = 0.75f * frac + 0.25f * ((float)ncand / (float)E);
 
         // ---- b. candidates: exact top-c ----
-        const int ce = min(cq, ncand);
-        if (ce > 0 && P == 1) {
+        const int ce = hash_mode ? ncand : min(cq, ncand);
+        if (hash_mode) {
+            // no selection: every candidate is reranked
+        } else if (ce > 0 && P == 1) {
             // Single pass: the tile holds each occupied slot's sum (kUnset: empty) and
             // keys_g its id. Keys are (sum desc, id asc). A histogram of the sums over
             // kSelBins equal-width bins between the smallest and largest sum (a monotone
@@ -627,7 +650,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     rp[u] = nullptr;
                     ids[u] = 0;
                     if (i < ce) {
-                        ids[u] = cidl[i];
+                        ids[u] = hash_mode ? idlist[i] : cidl[i];
                         int64_t slot = (int64_t)ids[u] + rows.head;
                         if (slot >= rows.cap) slot -= rows.cap;
                         rp[u] = rows.X + slot * d;
@@ -664,7 +687,12 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     float v = acc[u];
                     for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
                     const int i = base + ((u * kSWaves + wave) * groups + sub);
-                    if (i < ce && gl == 0) rk[i] = ckey(v, ids[u]);
+                    if (i < ce && gl == 0) {
+                        if (hash_mode)
+                            cand[i] = ckey(v, ids[u]);
+                        else
+                            rk[i] = ckey(v, ids[u]);
+                    }
                 }
             }
             __syncthreads();
@@ -674,7 +702,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         const int64_t off = qb.out_off ? qb.out_off[q] : (int64_t)q * qb.k0;
         const int kk = min(kq, ce);
         if (kk > 0) {
-            auto get = [&](int i) -> uint64_t { return rk[i]; };
+            auto get = [&](int i) -> uint64_t { return hash_mode ? cand[i] : rk[i]; };
             uint64_t prefix;
             int shift;
             radix_select<kSNT>(ce, kk, get, sh, prefix, shift);

@@ -74,6 +74,14 @@ int ceos_search_coceos(ceos_index* h, const float* Q, int nq, int d, int k, int3
 int ceos_search_coceos_device(ceos_index* h, const float* dQ, int nq, int d, int k, int32_t* d_ids,
                               float* d_ips, void* stream);
 
+/* build_CEOs_Hash(dataset) — python_wrapper.cpp:46, CEOs.cpp:709-902: the coCEOs lists
+ * (ids are what CEOs-Hash uses). With centering set, the stored points are X minus its
+ * column mean. search_CEOs_Hash(queries, n_neighbors) — python_wrapper.cpp:47-48,
+ * CEOs.cpp:904-1066: every distinct id among the first n_probed_points entries of the
+ * n_probed_vectors probed lists is reranked (n_cand is not used). */
+int ceos_build_hash(ceos_index* h, const float* X, int64_t n, int d);
+int ceos_search_hash(ceos_index* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips);
+
 /* Batched coCEOs search with per-query k[q], n_probed_vectors s[q] and n_cand c[q]
  * (no reference counterpart: the batched form of search_coCEOs_Est). Query q's
  * results are written at off[q] .. off[q]+k[q]-1 of ids/ips, where
@@ -222,6 +230,8 @@ int ceos_stream_update(ceos_stream* h, const float* Xnew, int64_t n_new, int d, 
 /* estimate_search(queries, n_neighbors) — python_wrapper.cpp:67-68, streamCEOs.cpp:378-612 */
 int ceos_stream_estimate_search(ceos_stream* h, const float* Q, int nq, int d, int k, int32_t* ids,
                                 float* ips);
+/* hash_search(queries, n_neighbors) — python_wrapper.cpp:69-70, streamCEOs.cpp:621-877 */
+int ceos_stream_hash_search(ceos_stream* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips);
 /* Random delete (no reference counterpart; the reference deletes FIFO only): the
  * points with these current ids leave every bucket at once and are never returned
  * again; ids of other points do not change. The rows keep their positions until the

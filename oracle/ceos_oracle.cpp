@@ -243,6 +243,31 @@ void search_one(const Params& P, const Buckets& B, const float* q, int k, int s,
     rerank(q, P.d, asc, row, k, out_id, out_ip);
 }
 
+// CEOs-Hash search (src/CEOs.cpp:904-1066; streamCEOs.cpp:621-877): the union of the
+// first mp ids of the probed lists (a bitset over the points), every member reranked in
+// increasing id order (bitset find_first/find_next) into a top-k heap with strict '>'.
+void hash_one(const Params& P, const Buckets& B, const float* q, int k, int s, int mp, int64_t n,
+              const std::function<const float*(int)>& row, int32_t* out_id, float* out_ip, int* cand_count) {
+    const int DR = P.D * P.R;
+    std::vector<float> proj(DR);
+    project_row(P, q, proj.data());
+    std::vector<int> probes = probe_order(proj.data(), DR, s);
+    std::vector<uint8_t> bits((size_t)n, 0);
+    for (int code : probes) {
+        const bool far = code < 0;
+        const int col = (far ? -code : code) - 1;
+        const int32_t* lid = far ? &B.fid[(size_t)col * B.m] : &B.cid[(size_t)col * B.m];
+        const int valid = far ? B.fsize[col] : B.csize[col];
+        const int lim = std::min(mp, valid);
+        for (int t = 0; t < lim; ++t) bits[lid[t]] = 1;
+    }
+    std::vector<int> asc;
+    for (int64_t i = 0; i < n; ++i)
+        if (bits[i]) asc.push_back((int)i);
+    if (cand_count) *cand_count = (int)asc.size();
+    rerank(q, P.d, asc, row, k, out_id, out_ip);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------
@@ -363,6 +388,23 @@ int oracle_ceos_search_coceos(void* hv, const float* Q, int nq, const int* k, co
 }
 
 // src/CEOs.cpp:18-98 (build_CEOs): keeps all n x DR projections.
+// search_CEOs_Hash (src/CEOs.cpp:904-1066); guards src/CEOs.cpp:906-915.
+int oracle_ceos_search_hash(void* hv, const float* Q, int nq, int k, int s, int mp, int32_t* out_id,
+                            float* out_ip, int* cand_count) {
+    auto* h = (OracleCEOs*)hv;
+    const Params& P = h->P;
+    if (mp > P.m || s > P.D * P.R) return 1;
+    const int d = P.d;
+    const float* X = h->X.data();
+    auto row = [X, d](int id) { return X + (size_t)id * d; };
+#pragma omp parallel for num_threads(P.threads) schedule(dynamic, 4)
+    for (int q = 0; q < nq; ++q)
+        hash_one(P, h->B, Q + (size_t)q * d, k, s, mp, (int64_t)(h->X.size() / d), row, out_id + (size_t)q * k,
+                 out_ip + (size_t)q * k,
+                 cand_count ? cand_count + q : nullptr);
+    return 0;
+}
+
 void oracle_ceos_build_est(void* hv, const float* X) {
     auto* h = (OracleCEOs*)hv;
     Params& P = h->P;
@@ -521,6 +563,21 @@ int oracle_stream_search(void* hv, const float* Q, int nq, int k, int s, int mp,
     for (int q = 0; q < nq; ++q)
         search_one(P, h->B, Q + (size_t)q * P.d, k, s, mp, c, row, out_id + (size_t)q * k,
                    out_ip + (size_t)q * k, nullptr);
+    return 0;
+}
+
+// hash_search (src/streamCEOs.cpp:621-877); guard s <= n_proj (src/streamCEOs.cpp:630).
+int oracle_stream_hash_search(void* hv, const float* Q, int nq, int k, int s, int mp, int32_t* out_id,
+                              float* out_ip) {
+    auto* h = (OracleStream*)hv;
+    const Params& P = h->P;
+    if (mp > P.m || s > P.D) return 1;
+    auto row = [h](int id) { return h->X[id].data(); };
+    const int64_t n = (int64_t)h->X.size();
+#pragma omp parallel for num_threads(P.threads) schedule(dynamic, 4)
+    for (int q = 0; q < nq; ++q)
+        hash_one(P, h->B, Q + (size_t)q * P.d, k, s, mp, n, row, out_id + (size_t)q * k, out_ip + (size_t)q * k,
+                 nullptr);
     return 0;
 }
 

@@ -66,6 +66,8 @@ def lib():
             "oracle_stream_get_buckets": (None, [vp, _i32p, _f32p, _i32p, _f32p, _i32p, _i32p]),
             "oracle_stream_size": (i32, [vp]),
             "oracle_stream_search": (i32, [vp, _f32p, i32, i32, i32, i32, i32, _i32p, _f32p]),
+            "oracle_ceos_search_hash": (i32, [vp, _f32p, i32, i32, i32, i32, _i32p, _f32p, _i32p]),
+            "oracle_stream_hash_search": (i32, [vp, _f32p, i32, i32, i32, i32, _i32p, _f32p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -193,6 +195,23 @@ class OracleCEOs:
             return ids, ips, distinct, margin
         return ids, ips
 
+    def build_CEOs_Hash(self, X):
+        """CEOs-Hash keeps the coCEOs lists' ids (src/CEOs.cpp:709-902); no centering."""
+        self.build_coCEOs_Est(X)
+
+    def search_CEOs_Hash(self, Q, n_neighbors, verbose=False, *, stats=False):
+        """src/CEOs.cpp:904-1066: rerank the union of the probed lists' first ids."""
+        Q = _f32(Q)
+        nq = Q.shape[0]
+        ids = np.zeros((nq, n_neighbors), np.int32)
+        ips = np.zeros((nq, n_neighbors), np.float32)
+        cnt = np.zeros(nq, np.int32)
+        rc = lib().oracle_ceos_search_hash(self._h, Q, nq, n_neighbors, self.n_probed_vectors,
+                                           self.n_probed_points, ids, ips, cnt)
+        if rc:
+            raise ValueError("invalid query parameters")
+        return (ids, ips, cnt) if stats else (ids, ips)
+
     def build(self, X):
         lib().oracle_ceos_build_est(self._h, _f32(X))
 
@@ -246,6 +265,18 @@ class OracleStream:
         cs = np.zeros(DR, np.int32); fs = np.zeros(DR, np.int32)
         lib().oracle_stream_get_buckets(self._h, cid, cv, fid, fv, cs, fs)
         return cid, cv, fid, fv, cs, fs
+
+    def hash_search(self, Q, n_neighbors, verbose=False):
+        """streamCEOs.hash_search (src/streamCEOs.cpp:621-877)."""
+        Q = _f32(Q)
+        nq = Q.shape[0]
+        ids = np.zeros((nq, n_neighbors), np.int32)
+        ips = np.zeros((nq, n_neighbors), np.float32)
+        rc = lib().oracle_stream_hash_search(self._h, Q, nq, n_neighbors, self.n_probed_vectors,
+                                             self.n_probed_points, ids, ips)
+        if rc:
+            raise ValueError("invalid query parameters")
+        return ids, ips
 
     def estimate_search(self, Q, n_neighbors, verbose=False):
         Q = _f32(Q)

@@ -207,6 +207,50 @@ def test_coceos_with_full_buckets_equals_ceos_est(oracle):
     assert np.array_equal(pa, pb)
 
 
+def test_hash_search_full_buckets_is_exact_mips(oracle):
+    """CEOs-Hash with every point in every bucket reranks every point: exact MIPS."""
+    n, d = 400, 20
+    rng = np.random.default_rng(61)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    Q = rng.standard_normal((15, d)).astype(np.float32)
+    o = oracle.OracleCEOs(n, d)
+    o.setIndexParam(32, 2, n, 1, 42)
+    o.build_CEOs_Hash(X)
+    o.n_probed_vectors, o.n_probed_points = 3, n
+    ids, ips, cnt = o.search_CEOs_Hash(Q, 8, stats=True)
+    assert np.all(cnt == n)
+    ex, _ = oracle.exact_mips(X, Q, 8)
+    assert np.array_equal(ids, ex)
+
+
+def test_hash_candidates_are_union_of_probed_lists(oracle):
+    """The candidate set is the union of the first m_p ids of the s probed lists (probes
+    recomputed from the oracle's projection), and the result is the exact top-k of it."""
+    n, d, D, R, m = 2000, 32, 64, 2, 60
+    rng = np.random.default_rng(62)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    Q = rng.standard_normal((12, d)).astype(np.float32)
+    o = oracle.OracleCEOs(n, d)
+    o.setIndexParam(D, R, m, 1, 42)
+    o.build_CEOs_Hash(X)
+    s, mp, k = 9, 40, 7
+    o.n_probed_vectors, o.n_probed_points = s, mp
+    ids, ips, cnt = o.search_CEOs_Hash(Q, k, stats=True)
+    cid, _, fid, _ = o.buckets()
+    P = o.project(Q)
+    for q in range(len(Q)):
+        top = np.argsort(-np.abs(P[q]), kind="stable")[:s]
+        union = set()
+        for col in top:
+            lst = fid[col] if P[q, col] < 0 else cid[col]
+            union.update(lst[:mp].tolist())
+        assert cnt[q] == len(union)
+        cand = np.array(sorted(union))
+        ip = (X[cand].astype(np.float64) @ Q[q].astype(np.float64))
+        order = cand[np.lexsort((cand, -ip))][:k]
+        assert np.array_equal(ids[q], order)
+
+
 def test_probe_order_and_scores_bruteforce(oracle):
     """Estimates recomputed in numpy from projections, probes and buckets pick the same candidates."""
     n, d, D, R, m, s, c, k = 800, 20, 64, 2, 40, 9, 60, 10
