@@ -260,8 +260,11 @@ struct Core {
     }
 
     void make_signs() {
-        seed_used = seed >= 0 ? (unsigned)seed
-                              : (unsigned)std::chrono::system_clock::now().time_since_epoch().count();
+        make_signs_from(seed >= 0 ? (unsigned)seed
+                                  : (unsigned)std::chrono::system_clock::now().time_since_epoch().count());
+    }
+    void make_signs_from(unsigned seed_value) {
+        seed_used = seed_value;
         bits = sign_bits(F * 3 * R, seed_used);
         const int W = (F + 31) / 32;
         std::vector<uint32_t> words((size_t)W * 3 * R, 0u);
@@ -441,6 +444,130 @@ struct Core {
         HIP_CHECK(hipMemcpyAsync(ips, dP.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, st()));
         HIP_CHECK(hipStreamSynchronize(st()));
         check_status();
+    }
+
+    // ---- index files (no reference counterpart): parameters, points in logical order,
+    // buckets with logical ids, the stream's random-delete flags. Little-endian binary:
+    //   "CEOSIDX1" | i32 kind, d, D, R, m, F, L, s, mp, c, centering | u32 seed |
+    //   i64 live | X [live][d] f32 | buckets [2 DR][m] (u32 id, f32 value) | sizes [2 DR] i32 |
+    //   delete flags [(live + 31) / 32] u32
+    static constexpr char kMagic[8] = {'C', 'E', 'O', 'S', 'I', 'D', 'X', '1'};
+    void save(const char* path, int kind) {
+        if (!built) throw StateError("index is not built");
+        FILE* f = std::fopen(path, "wb");
+        if (!f) throw ArgError(std::string("cannot open ") + path + " for writing");
+        auto put = [&](const void* p, size_t bytes) {
+            if (bytes && std::fwrite(p, 1, bytes, f) != bytes) {
+                std::fclose(f);
+                throw DevError(std::string("write to ") + path + " failed");
+            }
+        };
+        const int32_t hdr[11] = {kind, d, D, R, m, F, L, s, mp, c, centering ? 1 : 0};
+        const int64_t n = live;
+        put(kMagic, 8);
+        put(hdr, sizeof(hdr));
+        put(&seed_used, 4);
+        put(&n, 8);
+        // rows in logical order, through a bounded host buffer
+        const int64_t chunk = std::max<int64_t>(1, (int64_t(64) << 20) / (4LL * d));
+        std::vector<float> buf;
+        for (int64_t i = 0; i < n;) {
+            int64_t slot = (head + i) % cap;
+            const int64_t len = std::min({chunk, n - i, cap - slot});
+            buf.resize((size_t)len * d);
+            HIP_CHECK(hipMemcpy(buf.data(), X.p + slot * d, (size_t)len * d * 4, hipMemcpyDeviceToHost));
+            put(buf.data(), buf.size() * 4);
+            i += len;
+        }
+        const int nl = 2 * DR();
+        std::vector<Entry> h((size_t)nl * m);
+        std::vector<int32_t> sz(nl);
+        HIP_CHECK(hipMemcpy(h.data(), bkt.p, h.size() * sizeof(Entry), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(sz.data(), bsize.p, sz.size() * 4, hipMemcpyDeviceToHost));
+        for (auto& e : h) e.x -= base;  // logical ids
+        put(h.data(), h.size() * sizeof(Entry));
+        put(sz.data(), sz.size() * 4);
+        std::vector<uint32_t> flags((size_t)(n + 31) / 32, 0u);
+        if (dead.p) {
+            std::vector<uint32_t> ring((size_t)(cap + 31) / 32);
+            HIP_CHECK(hipMemcpy(ring.data(), dead.p, ring.size() * 4, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) {
+                const int64_t slot = (head + i) % cap;
+                if ((ring[slot >> 5] >> (slot & 31)) & 1u) flags[i >> 5] |= 1u << (i & 31);
+            }
+        }
+        put(flags.data(), flags.size() * 4);
+        if (std::fclose(f) != 0) throw DevError(std::string("closing ") + path + " failed");
+    }
+    // Returns the number of points.
+    int64_t load(const char* path, int kind) {
+        FILE* f = std::fopen(path, "rb");
+        if (!f) throw ArgError(std::string("cannot open ") + path);
+        auto get = [&](void* p, size_t bytes) {
+            if (bytes && std::fread(p, 1, bytes, f) != bytes) {
+                std::fclose(f);
+                throw ArgError(std::string(path) + ": truncated index file");
+            }
+        };
+        char magic[8];
+        int32_t hdr[11];
+        uint32_t seed_v;
+        int64_t n;
+        get(magic, 8);
+        if (std::memcmp(magic, kMagic, 8) != 0) {
+            std::fclose(f);
+            throw ArgError(std::string(path) + " is not an index file");
+        }
+        get(hdr, sizeof(hdr));
+        get(&seed_v, 4);
+        get(&n, 8);
+        if (hdr[0] != kind) {
+            std::fclose(f);
+            throw ArgError(std::string(path) + (kind ? " holds a CEOs index, not a streamCEOs one"
+                                                     : " holds a streamCEOs index, not a CEOs one"));
+        }
+        if (hdr[1] != d) {
+            std::fclose(f);
+            throw ArgError("the index file has " + std::to_string(hdr[1]) + " features, the handle " +
+                           std::to_string(d));
+        }
+        built = false;
+        D = hdr[2]; R = hdr[3]; m = hdr[4]; F = hdr[5]; L = hdr[6];
+        s = hdr[7]; mp = hdr[8]; c = hdr[9]; centering = hdr[10] != 0;
+        seed = (int)(seed_v & 0x7fffffffu) == (int)seed_v ? (int)seed_v : -1;
+        params_set = true;
+        estP.release();
+        X.alloc((size_t)std::max<int64_t>(n, 1) * d);
+        cap = std::max<int64_t>(n, 1); head = 0; base = 0; live = (uint32_t)n;
+        const int64_t chunk = std::max<int64_t>(1, (int64_t(64) << 20) / (4LL * d));
+        std::vector<float> buf;
+        for (int64_t i = 0; i < n; i += chunk) {
+            const int64_t len = std::min(chunk, n - i);
+            buf.resize((size_t)len * d);
+            get(buf.data(), buf.size() * 4);
+            HIP_CHECK(hipMemcpy(X.p + i * d, buf.data(), buf.size() * 4, hipMemcpyHostToDevice));
+        }
+        const int nl = 2 * DR();
+        std::vector<Entry> h((size_t)nl * m);
+        std::vector<int32_t> sz(nl);
+        std::vector<uint32_t> flags((size_t)(n + 31) / 32);
+        get(h.data(), h.size() * sizeof(Entry));
+        get(sz.data(), sz.size() * 4);
+        get(flags.data(), flags.size() * 4);
+        std::fclose(f);
+        bkt.alloc(h.size());
+        bsize.alloc(sz.size());
+        HIP_CHECK(hipMemcpy(bkt.p, h.data(), h.size() * sizeof(Entry), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(bsize.p, sz.data(), sz.size() * 4, hipMemcpyHostToDevice));
+        dead.release();
+        if (kind == 1) {  // streams always carry the flags (ring slots = logical rows here)
+            flags.resize((size_t)(cap + 31) / 32, 0u);
+            dead.alloc(flags.size());
+            HIP_CHECK(hipMemcpy(dead.p, flags.data(), flags.size() * 4, hipMemcpyHostToDevice));
+        }
+        make_signs_from(seed_v);
+        built = true;
+        return n;
     }
 
     void get_buckets(int32_t* cid, float* cval, int32_t* fid, float* fval, int32_t* csz, int32_t* fsz) {

@@ -199,6 +199,23 @@ class PyCEOs {
 
     // Batched form: per-query k, n_probed_vectors and n_cand. Returns (ids, ips, offsets)
     // with query q's results at offsets[q] .. offsets[q+1]-1.
+    void save(const std::string& path) const {
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_save(h_, path.c_str());
+        }
+        check(rc);
+    }
+    void load(const std::string& path) {
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_load(h_, path.c_str());
+        }
+        check(rc);
+    }
+
     PyPlan* make_plan(int nq_max, int k_max, int s_max, int c_max, bool autotune) {
         ceos_plan* p = nullptr;
         check(ceos_plan_create(h_, nq_max, k_max, s_max, c_max, autotune ? CEOS_PLAN_AUTOTUNE : 0, &p));
@@ -315,6 +332,23 @@ class PyStream {
 
     ceos_stream* raw() const { return h_; }
     int dim() const { return d_; }
+
+    void save(const std::string& path) const {
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_stream_save(h_, path.c_str());
+        }
+        check(rc);
+    }
+    void load(const std::string& path) {
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_stream_load(h_, path.c_str());
+        }
+        check(rc);
+    }
 
     PyPlan* make_plan(int nq_max, int k_max, int s_max, int c_max, bool autotune) {
         ceos_plan* p = nullptr;
@@ -559,6 +593,8 @@ PYBIND11_MODULE(CEOs, m) {
         .def("build_CEOs_Hash", &PyCEOs::build_CEOs_Hash, py::arg("dataset"))
         .def("search_CEOs_Hash", &PyCEOs::search_CEOs_Hash, py::arg("queries"), py::arg("n_neighbors"),
              py::arg("verbose") = false)
+        .def("save", &PyCEOs::save, py::arg("path"))
+        .def("load", &PyCEOs::load, py::arg("path"))
         .def("make_plan", &PyCEOs::make_plan, py::arg("nq_max"), py::arg("k_max"), py::arg("s_max"),
              py::arg("c_max"), py::arg("autotune") = true, py::keep_alive<0, 1>())
         .def("search_batched", &PyCEOs::search_batched, py::arg("queries"), py::arg("n_neighbors"),
@@ -613,6 +649,8 @@ PYBIND11_MODULE(CEOs, m) {
         .def("set_threads", &PyStream::set_threads, py::arg("n_threads"))
         .def("clear", &PyStream::clear)
         .def("build", &PyStream::build, py::arg("dataset"))
+        .def("save", &PyStream::save, py::arg("path"))
+        .def("load", &PyStream::load, py::arg("path"))
         .def("make_plan", &PyStream::make_plan, py::arg("nq_max"), py::arg("k_max"), py::arg("s_max"),
              py::arg("c_max"), py::arg("autotune") = true, py::keep_alive<0, 1>())
         .def("update", &PyStream::update, py::arg("new_dataset"), py::arg("n_delPoints") = 0)

@@ -146,6 +146,24 @@ int ceos_search_hash(ceos_index* h, const float* Q, int nq, int d, int k, int32_
     });
 }
 
+int ceos_save(const ceos_index* h, const char* path) {
+    return guarded([&] {
+        if (!h || !path) throw ArgError("null argument");
+        auto& c = const_cast<Core&>(h->core);
+        std::lock_guard<std::mutex> g(c.mu);
+        if (c.estP.p && !c.bkt.p) throw StateError("CEOs-Est indexes are not saved; build coCEOs or CEOs-Hash");
+        c.save(path, 0);
+    });
+}
+
+int ceos_load(ceos_index* h, const char* path) {
+    return guarded([&] {
+        if (!h || !path) throw ArgError("null argument");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->n_points = (int)h->core.load(path, 0);
+    });
+}
+
 int ceos_search_coceos(ceos_index* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips) {
     return guarded([&] {
         if (!h) throw ArgError("null handle");
@@ -573,6 +591,23 @@ int ceos_stream_hash_search(ceos_stream* h, const float* Q, int nq, int d, int k
         check_matrix(Q, nq, d, c.d);
         c.validate_query(c.s, c.mp, 1, k, c.D, "number of projections");  // src/streamCEOs.cpp:623-633
         c.search_host(Q, nq, k, c.s, 1, ids, ips, true);
+    });
+}
+
+int ceos_stream_save(const ceos_stream* h, const char* path) {
+    return guarded([&] {
+        if (!h || !path) throw ArgError("null argument");
+        auto& c = const_cast<Core&>(h->core);
+        std::lock_guard<std::mutex> g(c.mu);
+        c.save(path, 1);
+    });
+}
+
+int ceos_stream_load(ceos_stream* h, const char* path) {
+    return guarded([&] {
+        if (!h || !path) throw ArgError("null argument");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        (void)h->core.load(path, 1);
     });
 }
 

@@ -82,6 +82,14 @@ int ceos_search_coceos_device(ceos_index* h, const float* dQ, int nq, int d, int
 int ceos_build_hash(ceos_index* h, const float* X, int64_t n, int d);
 int ceos_search_hash(ceos_index* h, const float* Q, int nq, int d, int k, int32_t* ids, float* ips);
 
+/* Index files (no reference counterpart; the reference rebuilds on every start): the
+ * parameters, the stored points, the buckets and the sign seed of a built coCEOs /
+ * CEOs-Hash index, so that a restart skips the build. load replaces the handle's index;
+ * the file must hold the handle's n_features. Searches after load return what the saved
+ * index returned. */
+int ceos_save(const ceos_index* h, const char* path);
+int ceos_load(ceos_index* h, const char* path);
+
 /* Batched coCEOs search with per-query k[q], n_probed_vectors s[q] and n_cand c[q]
  * (no reference counterpart: the batched form of search_coCEOs_Est). Query q's
  * results are written at off[q] .. off[q]+k[q]-1 of ids/ips, where
@@ -238,6 +246,9 @@ int ceos_stream_hash_search(ceos_stream* h, const float* Q, int nq, int d, int k
  * FIFO delete of update() drops them. */
 int ceos_stream_delete_ids(ceos_stream* h, const int32_t* ids, int64_t count);
 int ceos_stream_size(const ceos_stream* h, int64_t* n_points);
+/* streamCEOs index files: live points in id order, buckets, random-delete marks. */
+int ceos_stream_save(const ceos_stream* h, const char* path);
+int ceos_stream_load(ceos_stream* h, const char* path);
 /* Buckets with per-list valid sizes ([n_proj*n_repeats] each). */
 int ceos_stream_get_buckets(const ceos_stream* h, int32_t* close_ids, float* close_vals, int32_t* far_ids,
                             float* far_vals, int32_t* close_size, int32_t* far_size);
