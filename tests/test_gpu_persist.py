@@ -64,3 +64,25 @@ def test_save_load_stream(CEOs, oracle, tmp_path):
     a = g.estimate_search(Q, 10)
     b = r.estimate_search(Q, 10)
     assert np.array_equal(a[0], b[0]) and np.array_equal(bits(a[1]), bits(b[1]))
+
+
+def test_cli_end_to_end(CEOs, tmp_path):
+    """python -m ceos_amd.cli on .bin files returns what the module returns."""
+    from ceos_amd.cli import parse, run
+    from ceos_amd.io import save_bin
+
+    X = _rand(3000, 20, 95)
+    Q = _rand(12, 20, 96)
+    px, pq = str(tmp_path / "x.bin"), str(tmp_path / "q.bin")
+    save_bin(px, X)
+    save_bin(pq, Q)
+    a = parse(f"--X {px} --Q {pq} --n_points 3000 --n_features 20 --n_queries 12 --n_proj 64 --n_repeats 2 "
+              f"--top_m 100 --random_seed 3 --n_neighbors 5 --n_probed_vectors 8 --n_probed_points 50 "
+              f"--n_cand 60 --method coceos,hash".split())
+    out = run(a)
+    g = CEOs.CEOs(3000, 20)
+    g.setIndexParam(64, 2, 100, -1, 3)
+    g.build_coCEOs_Est(X)
+    g.n_probed_vectors, g.n_probed_points, g.n_cand = 8, 50, 60
+    assert np.array_equal(out["ids_search_coCEOs_Est"], g.search_coCEOs_Est(Q, 5)[0])
+    assert np.array_equal(out["ids_search_CEOs_Hash"], g.search_CEOs_Hash(Q, 5)[0])
