@@ -182,6 +182,7 @@ struct Core {
     hipStream_t st_ = nullptr;
 
     std::vector<uint8_t> bits;
+    std::vector<float> center;  // streamCEOs with centering: column mean of the build rows
     DBuf<uint32_t> signs;
     DBuf<Entry> bkt;
     DBuf<int32_t> bsize;
@@ -450,7 +451,7 @@ struct Core {
     // buckets with logical ids, the stream's random-delete flags. Little-endian binary:
     //   "CEOSIDX1" | i32 kind, d, D, R, m, F, L, s, mp, c, centering | u32 seed |
     //   i64 live | X [live][d] f32 | buckets [2 DR][m] (u32 id, f32 value) | sizes [2 DR] i32 |
-    //   delete flags [(live + 31) / 32] u32
+    //   delete flags [(live + 31) / 32] u32 | stream with centering: center [d] f32
     static constexpr char kMagic[8] = {'C', 'E', 'O', 'S', 'I', 'D', 'X', '1'};
     void save(const char* path, int kind) {
         if (!built) throw StateError("index is not built");
@@ -497,6 +498,7 @@ struct Core {
             }
         }
         put(flags.data(), flags.size() * 4);
+        if (kind == 1 && centering) put(center.data(), (size_t)d * 4);  // the stream's fixed center
         if (std::fclose(f) != 0) throw DevError(std::string("closing ") + path + " failed");
     }
     // Returns the number of points.
@@ -554,6 +556,11 @@ struct Core {
         get(h.data(), h.size() * sizeof(Entry));
         get(sz.data(), sz.size() * 4);
         get(flags.data(), flags.size() * 4);
+        center.clear();
+        if (kind == 1 && centering) {
+            center.resize(d);
+            get(center.data(), (size_t)d * 4);
+        }
         std::fclose(f);
         bkt.alloc(h.size());
         bsize.alloc(sz.size());

@@ -331,6 +331,8 @@ class PyStream {
     ~PyStream() { ceos_stream_destroy(h_); }
 
     ceos_stream* raw() const { return h_; }
+    bool get_centering() const { int v; check(ceos_stream_get_centering(h_, &v)); return v != 0; }
+    void set_centering(bool v) { check(ceos_stream_set_centering(h_, v ? 1 : 0)); }
     int dim() const { return d_; }
 
     void save(const std::string& path) const {
@@ -646,6 +648,7 @@ PYBIND11_MODULE(CEOs, m) {
         .def_property("n_probed_vectors", &PyStream::get_s, &PyStream::set_s)
         .def_property("n_probed_points", &PyStream::get_mp, &PyStream::set_mp)
         .def_property("n_cand", &PyStream::get_c, &PyStream::set_c)
+        .def_property("centering", &PyStream::get_centering, &PyStream::set_centering)
         .def("set_threads", &PyStream::set_threads, py::arg("n_threads"))
         .def("clear", &PyStream::clear)
         .def("build", &PyStream::build, py::arg("dataset"))

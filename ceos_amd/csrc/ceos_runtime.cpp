@@ -435,6 +435,22 @@ int ceos_stream_get_query_param(const ceos_stream* h, int* s, int* mp, int* c) {
     });
 }
 
+// centering (streamCEOs.h:44): the reference has the field but does not bind it
+// (python_wrapper.cpp:52-71); it takes effect at the next build.
+int ceos_stream_set_centering(ceos_stream* h, int centering) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.centering = centering != 0;
+    });
+}
+int ceos_stream_get_centering(const ceos_stream* h, int* centering) {
+    return guarded([&] {
+        if (!h || !centering) throw ArgError("null argument");
+        *centering = h->core.centering ? 1 : 0;
+    });
+}
+
 int ceos_stream_set_threads(ceos_stream* h, int n_threads) {
     return guarded([&] { if (!h) throw ArgError("null handle"); h->core.threads = n_threads; });
 }
@@ -458,6 +474,21 @@ int ceos_stream_build(ceos_stream* h, const float* X, int64_t n, int d) {
         if (n >= (int64_t(1) << 31)) throw ArgError("more than 2^31-1 points");
         c.built = false;
         const int64_t capacity = std::max<int64_t>(n + n / 4, 1024);
+        // centering: rows are stored minus the build rows' column mean, which stays
+        // fixed for later updates (src/streamCEOs.cpp:22-34, 198-203)
+        std::vector<float> Xc;
+        c.center.clear();
+        if (c.centering) {
+            std::vector<double> mean(d, 0.0);
+            for (int64_t i = 0; i < n; ++i)
+                for (int j = 0; j < d; ++j) mean[j] += X[i * d + j];
+            c.center.resize(d);
+            for (int j = 0; j < d; ++j) c.center[j] = (float)(mean[j] / (double)n);
+            Xc.resize((size_t)n * d);
+            for (int64_t i = 0; i < n; ++i)
+                for (int j = 0; j < d; ++j) Xc[i * d + j] = X[i * d + j] - c.center[j];
+            X = Xc.data();
+        }
         c.load_points(X, n, capacity, false, c.st());
         c.dead.alloc((size_t)(capacity + 31) / 32);
         HIP_CHECK(hipMemsetAsync(c.dead.p, 0, c.dead.n * 4, c.st()));
@@ -478,6 +509,13 @@ int ceos_stream_update(ceos_stream* h, const float* Xn, int64_t n_new, int d, in
         if (n_del < 0) throw ArgError("n_delPoints must be >= 0");
         if (n_del > n_cur) throw ArgError("Number of removed points must be smaller than the current number of points");
         if (c.m > n_cur + n_new - n_del) throw ArgError("There is not enough indexed top-points for coCEOs after update");
+        std::vector<float> Xc;
+        if (c.centering && !c.center.empty() && n_new > 0) {  // src/streamCEOs.cpp:198-203
+            Xc.resize((size_t)n_new * d);
+            for (int64_t i = 0; i < n_new; ++i)
+                for (int j = 0; j < d; ++j) Xc[i * d + j] = Xn[i * d + j] - c.center[j];
+            Xn = Xc.data();
+        }
         if (n_cur + n_new >= (int64_t(1) << 31)) throw ArgError("more than 2^31-1 live points");
         hipStream_t st = c.st();
         // grow the ring when the new rows do not fit

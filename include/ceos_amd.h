@@ -228,6 +228,11 @@ int ceos_stream_set_query_param(ceos_stream* h, int n_probed_vectors, int n_prob
 int ceos_stream_get_query_param(const ceos_stream* h, int* n_probed_vectors, int* n_probed_points,
                                 int* n_cand);
 int ceos_stream_set_threads(ceos_stream* h, int n_threads);
+/* centering (streamCEOs.h:44, not bound in python_wrapper.cpp:52-71): stored rows are
+ * minus the build rows' column mean, fixed for later updates (streamCEOs.cpp:22-34,
+ * 198-203); queries are not centered. Takes effect at the next build. */
+int ceos_stream_set_centering(ceos_stream* h, int centering);
+int ceos_stream_get_centering(const ceos_stream* h, int* centering);
 int ceos_stream_clear(ceos_stream* h);
 
 /* build(dataset) — python_wrapper.cpp:65, streamCEOs.cpp:9-161 */

@@ -118,3 +118,30 @@ def test_stream_random_delete(CEOs, oracle):
     # a FIFO update afterwards still agrees with the closed form of the survivors
     g.update(_rand(500, d, 18), 100)
     assert g.size() == 3400
+
+
+def test_stream_centering_equals_precentered_data(CEOs):
+    """centering stores rows minus the build rows' column mean, fixed for updates
+    (src/streamCEOs.cpp:22-34, 198-203): the same as feeding pre-centered rows."""
+    d, D, R, m = 24, 64, 2, 30
+    X = _rand(6000, d, 19) + 2.5
+    Q = _rand(30, d, 20)
+    c = X[:3000].astype(np.float64).mean(axis=0).astype(np.float32)
+    a = CEOs.streamCEOs(d)
+    a.centering = True
+    assert a.centering
+    b = CEOs.streamCEOs(d)
+    for g in (a, b):
+        g.setIndexParam(D, R, m, -1, 42)
+    a.build(X[:3000])
+    b.build((X[:3000] - c).astype(np.float32))
+    for g in (a, b):
+        g.n_probed_vectors, g.n_probed_points, g.n_cand = 10, m, 100
+    ia, pa = a.estimate_search(Q, 8)
+    ib, pb = b.estimate_search(Q, 8)
+    assert np.array_equal(ia, ib) and np.array_equal(pa, pb)
+    a.update(X[3000:4500], 1000)
+    b.update((X[3000:4500] - c).astype(np.float32), 1000)
+    ia, pa = a.estimate_search(Q, 8)
+    ib, pb = b.estimate_search(Q, 8)
+    assert np.array_equal(ia, ib) and np.array_equal(pa, pb)
