@@ -237,7 +237,7 @@ struct SelectShared {
 // Keys are read 8 per thread at a time (independent loads in flight) and equal
 // consecutive digits of one thread are added to the histogram as one atomic.
 template <int NT, typename GetKey, bool kSkipZero = false>
-__device__ void radix_select(int n, int K, GetKey get, SelectShared& sh, uint64_t& prefix_out, int& shift_out,
+__device__ __forceinline__ void radix_select(int n, int K, GetKey get, SelectShared& sh, uint64_t& prefix_out, int& shift_out,
                              int min_shift = 0) {
     constexpr int UNR = 8;
     const int tid = threadIdx.x;
@@ -353,7 +353,7 @@ __device__ __forceinline__ uint32_t wave_append_n(uint32_t cnt, uint32_t* counte
 
 // Compacts the keys selected by (prefix, shift) into out[] (order unspecified).
 template <int NT, typename GetKey, typename Emit>
-__device__ void compact_selected(int n, GetKey get, uint64_t prefix, int shift, uint32_t* counter, Emit emit) {
+__device__ __forceinline__ void compact_selected(int n, GetKey get, uint64_t prefix, int shift, uint32_t* counter, Emit emit) {
     for (int i0 = 0; i0 < n; i0 += NT) {
         const int i = i0 + (int)threadIdx.x;
         uint64_t k = 0;
@@ -369,7 +369,7 @@ __device__ void compact_selected(int n, GetKey get, uint64_t prefix, int shift, 
 
 // Descending bitonic sort of N (power of two) 64-bit keys in LDS.
 template <int NT>
-__device__ void bitonic_sort_desc(uint64_t* s, int N) {
+__device__ __forceinline__ void bitonic_sort_desc(uint64_t* s, int N) {
     const int tid = threadIdx.x;
     for (int size = 2; size <= N; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {

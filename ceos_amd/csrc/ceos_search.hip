@@ -41,6 +41,9 @@ constexpr int kPNT = 256;              // probe-select block
 constexpr uint32_t kEmptyKey = 0xffffffffu;
 constexpr int kBufEntries = 8;         // bucket entries per thread per chunk
 constexpr uint32_t kUnset = 0xffffffffu;       // sum bits of a slot before its first (ordered) add
+typedef const float __attribute__((address_space(1)))* GlobalF;
+typedef float F4v __attribute__((ext_vector_type(4)));
+typedef const F4v __attribute__((address_space(1)))* GlobalF4;
 constexpr uint32_t kStashMark = 0xfffffffeu;   // phase-1 slot value: entry sent to the stash
 constexpr uint32_t kToPlace = 0xfffffffdu;     // phase-1 slot value: entry not placed yet
 constexpr int kEmitU = 8;                      // tile slots per thread per emission step
@@ -99,18 +102,28 @@ int search_lds_bytes(int d, int table_log2, int s_max, int c_max, int k_max) {
 }
 
 // ---------------------------------------------------------------------------------
+// STAGE: the query's DR projections are copied to LDS once (after the keys) and the radix
+// passes read them there; without it (DR above kProbeStageMax) they read global memory.
+constexpr int kProbeStageMax = 12288;
+template <bool STAGE>
 __global__ __launch_bounds__(kPNT) void k_probe_select(int DR, const float* __restrict__ qproj, int nq,
                                                        const int32_t* __restrict__ nq_dev,
                                                        const int32_t* __restrict__ s_arr, int s0, int s_stride,
                                                        int32_t* __restrict__ probes) {
     __shared__ SelectShared sh;
-    extern __shared__ uint64_t pkey[];
+    extern __shared__ uint64_t pkey[];  // next_pow2(s_stride) keys, then (STAGE) the DR projections
+    float* pl = reinterpret_cast<float*>(pkey + next_pow2(s_stride));
     if (nq_dev) nq = min(nq, nq_dev[0]);
     for (int q = blockIdx.x; q < nq; q += gridDim.x) {
         const int sq = min(max(s_arr ? s_arr[q] : s0, 0), min(s_stride, DR));
         if (sq == 0) continue;  // uniform
         const float* pq = qproj + (int64_t)q * DR;
-        auto get = [&](int c) -> uint64_t { return ckey(fabsf(pq[c]), (uint32_t)c); };
+        if (STAGE) {
+            for (int c = threadIdx.x; c < DR; c += kPNT) pl[c] = pq[c];
+            __syncthreads();
+        }
+        const float* src = STAGE ? pl : pq;
+        auto get = [&](int c) -> uint64_t { return ckey(fabsf(src[c]), (uint32_t)c); };
         uint64_t prefix;
         int shift;
         radix_select<kPNT>(DR, sq, get, sh, prefix, shift);
@@ -123,7 +136,7 @@ __global__ __launch_bounds__(kPNT) void k_probe_select(int DR, const float* __re
         bitonic_sort_desc<kPNT>(pkey, N);
         for (int j = threadIdx.x; j < sq; j += kPNT) {
             const int c = (int)ckey_id(pkey[sq - 1 - j]);
-            probes[(int64_t)q * s_stride + j] = 2 * c + (pq[c] < 0.0f ? 1 : 0);
+            probes[(int64_t)q * s_stride + j] = 2 * c + (src[c] < 0.0f ? 1 : 0);
         }
         __syncthreads();
     }
@@ -601,15 +614,20 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             for (int p2 = tid; p2 < above; p2 += kSNT) cidl[p2] = keys_g[cidl[p2]];
             __syncthreads();
             {
-                const bool in_lds = nb <= kBoundMax;
-                auto get = [&](int i) -> uint64_t { return in_lds ? bkeys[i] : cand[i]; };
-                uint64_t prefix;
-                int shift;
-                radix_select<kSNT>(nb, needb, get, sh, prefix, shift);
-                if (tid == 0) sh.counter = (uint32_t)above;
-                __syncthreads();
-                compact_selected<kSNT>(nb, get, prefix, shift, &sh.counter,
-                                       [&](uint32_t pos, uint64_t k) { cidl[pos] = ckey_id(k); });
+                // one instantiation per key array, so each reads one address space
+                auto bsel = [&](auto get) __attribute__((always_inline)) {
+                    uint64_t prefix;
+                    int shift;
+                    radix_select<kSNT>(nb, needb, get, sh, prefix, shift);
+                    if (tid == 0) sh.counter = (uint32_t)above;
+                    __syncthreads();
+                    compact_selected<kSNT>(nb, get, prefix, shift, &sh.counter,
+                                           [&](uint32_t pos, uint64_t k) { cidl[pos] = ckey_id(k); });
+                };
+                if (nb <= kBoundMax)
+                    bsel([&](int i) -> uint64_t { return bkeys[i]; });
+                else
+                    bsel([&](int i) -> uint64_t { return cand[i]; });
             }
             __syncthreads();
         } else if (ce > 0) {
@@ -618,19 +636,26 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 for (int i = tid; i < ncand; i += kSNT) tile64[i] = cand[i];
                 __syncthreads();
             }
-            auto get = [&](int i) -> uint64_t { return lds_keys ? tile64[i] : cand[i]; };
-            uint64_t prefix;
-            int shift;
-            radix_select<kSNT>(ncand, ce, get, sh, prefix, shift);
-            if (tid == 0) sh.counter = 0;
-            __syncthreads();
-            compact_selected<kSNT>(ncand, get, prefix, shift, &sh.counter,
-                                   [&](uint32_t pos, uint64_t k) { cidl[pos] = ckey_id(k); });
+            auto msel = [&](auto get) __attribute__((always_inline)) {
+                uint64_t prefix;
+                int shift;
+                radix_select<kSNT>(ncand, ce, get, sh, prefix, shift);
+                if (tid == 0) sh.counter = 0;
+                __syncthreads();
+                compact_selected<kSNT>(ncand, get, prefix, shift, &sh.counter,
+                                       [&](uint32_t pos, uint64_t k) { cidl[pos] = ckey_id(k); });
+            };
+            if (lds_keys)
+                msel([&](int i) -> uint64_t { return tile64[i]; });
+            else
+                msel([&](int i) -> uint64_t { return cand[i]; });
             __syncthreads();
         }
 
         // ---- c. rerank ----
-        {
+        // instantiated once per mode: ids and keys are in LDS (cidl, rk) or in global
+        // scratch (idlist, cand), and each copy reads one address space
+        auto rerank = [&](const uint32_t* idsrc, uint64_t* keydst) __attribute__((always_inline)) {
             const int d = dm.d;
             const bool vec = (d & 3) == 0;
             const int units = vec ? (d >> 2) : d;
@@ -641,7 +666,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             const int per_iter = kSWaves * groups * kRerankU;
             for (int base = 0; base < ce; base += per_iter) {
                 float acc[kRerankU];
-                const float* rp[kRerankU];
+                GlobalF rp[kRerankU];  // rows are in global memory: plain global loads, never flat
                 uint32_t ids[kRerankU];
 #pragma unroll
                 for (int u = 0; u < kRerankU; ++u) {
@@ -650,10 +675,10 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     rp[u] = nullptr;
                     ids[u] = 0;
                     if (i < ce) {
-                        ids[u] = hash_mode ? idlist[i] : cidl[i];
+                        ids[u] = idsrc[i];
                         int64_t slot = (int64_t)ids[u] + rows.head;
                         if (slot >= rows.cap) slot -= rows.cap;
-                        rp[u] = rows.X + slot * d;
+                        rp[u] = (GlobalF)(rows.X + slot * d);
                     }
                 }
                 if (vec) {
@@ -661,8 +686,10 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     for (int uu = gl; uu < units; uu += G) {
                         float4 a[kRerankU];
 #pragma unroll
-                        for (int u = 0; u < kRerankU; ++u)
-                            a[u] = rp[u] ? reinterpret_cast<const float4*>(rp[u])[uu] : make_float4(0, 0, 0, 0);
+                        for (int u = 0; u < kRerankU; ++u) {
+                            const F4v t = rp[u] ? reinterpret_cast<GlobalF4>(rp[u])[uu] : F4v{0.0f, 0.0f, 0.0f, 0.0f};
+                            a[u] = make_float4(t.x, t.y, t.z, t.w);
+                        }
                         const float4 b = q4[uu];
 #pragma unroll
                         for (int u = 0; u < kRerankU; ++u) {
@@ -687,33 +714,38 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     float v = acc[u];
                     for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
                     const int i = base + ((u * kSWaves + wave) * groups + sub);
-                    if (i < ce && gl == 0) {
-                        if (hash_mode)
-                            cand[i] = ckey(v, ids[u]);
-                        else
-                            rk[i] = ckey(v, ids[u]);
-                    }
+                    if (i < ce && gl == 0) keydst[i] = ckey(v, ids[u]);
                 }
             }
             __syncthreads();
-        }
+        };
+        if (hash_mode)
+            rerank(idlist, cand);
+        else
+            rerank(cidl, rk);
 
         // ---- d. top-k ----
         const int64_t off = qb.out_off ? qb.out_off[q] : (int64_t)q * qb.k0;
         const int kk = min(kq, ce);
         if (kk > 0) {
-            auto get = [&](int i) -> uint64_t { return hash_mode ? cand[i] : rk[i]; };
-            uint64_t prefix;
-            int shift;
-            radix_select<kSNT>(ce, kk, get, sh, prefix, shift);
-            const int N = next_pow2(kk);
-            for (int i = tid; i < N; i += kSNT) tk[i] = 0;
-            if (tid == 0) sh.counter = 0;
-            __syncthreads();
-            compact_selected<kSNT>(ce, get, prefix, shift, &sh.counter,
-                                   [&](uint32_t pos, uint64_t k) { tk[pos] = k; });
-            __syncthreads();
-            bitonic_sort_desc<kSNT>(tk, N);
+            // one instantiation per key array (LDS rk, or global cand in hash mode)
+            auto topk = [&](auto get) __attribute__((always_inline)) {
+                uint64_t prefix;
+                int shift;
+                radix_select<kSNT>(ce, kk, get, sh, prefix, shift);
+                const int N = next_pow2(kk);
+                for (int i = tid; i < N; i += kSNT) tk[i] = 0;
+                if (tid == 0) sh.counter = 0;
+                __syncthreads();
+                compact_selected<kSNT>(ce, get, prefix, shift, &sh.counter,
+                                       [&](uint32_t pos, uint64_t k) { tk[pos] = k; });
+                __syncthreads();
+                bitonic_sort_desc<kSNT>(tk, next_pow2(kk));
+            };
+            if (hash_mode)
+                topk([&](int i) -> uint64_t { return cand[i]; });
+            else
+                topk([&](int i) -> uint64_t { return rk[i]; });
         }
         const int kw = qb.out_off ? kq : qb.k0;  // fixed-stride rows are zero padded to the stride
         for (int t = tid; t < kw; t += kSNT) {
@@ -743,11 +775,17 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
     if (err != hipSuccess) return err;
     {
         const int pgrid = std::min(qb.nq, 4096);
-        const size_t plds = (size_t)next_pow2(cfg.s_max) * 8;
-        allow_lds_s(k_probe_select, plds);
-        hipLaunchKernelGGL(k_probe_select, dim3(pgrid), dim3(kPNT), plds, st, DR, scr.qproj, qb.nq, qb.nq_dev,
-                           qb.s, qb.s0,
-                           cfg.s_max, scr.probes);
+        const size_t klds = (size_t)next_pow2(cfg.s_max) * 8;
+        if (DR <= kProbeStageMax) {
+            const size_t plds = klds + (size_t)DR * 4;
+            allow_lds_s(k_probe_select<true>, plds);
+            hipLaunchKernelGGL(k_probe_select<true>, dim3(pgrid), dim3(kPNT), plds, st, DR, scr.qproj, qb.nq,
+                               qb.nq_dev, qb.s, qb.s0, cfg.s_max, scr.probes);
+        } else {
+            allow_lds_s(k_probe_select<false>, klds);
+            hipLaunchKernelGGL(k_probe_select<false>, dim3(pgrid), dim3(kPNT), klds, st, DR, scr.qproj, qb.nq,
+                               qb.nq_dev, qb.s, qb.s0, cfg.s_max, scr.probes);
+        }
         err = hipGetLastError();
         if (err != hipSuccess) return err;
     }

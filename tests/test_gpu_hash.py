@@ -97,3 +97,22 @@ def test_stream_hash_search(CEOs, oracle):
     g.n_probed_vectors = D + 1  # s <= n_proj (src/streamCEOs.cpp:630)
     with pytest.raises(ValueError):
         g.hash_search(Q, 10)
+
+
+def test_hash_search_multi_pass_small_tile(CEOs, oracle):
+    """A 256-slot tile forces several hash partitions per query (and stash use)."""
+    X = _rand(8000, 32, 89)
+    Q = _rand(40, 32, 90)
+    g = CEOs.CEOs(*X.shape)
+    g.setIndexParam(128, 2, 200, -1, 42)
+    g.build_CEOs_Hash(X)
+    g.n_probed_vectors, g.n_probed_points = 20, 200
+    o = oracle.OracleCEOs(*X.shape)
+    o.setIndexParam(128, 2, 200, 8, 42)
+    o.build_CEOs_Hash(X)
+    o.n_probed_vectors, o.n_probed_points = 20, 200
+    oi, op = o.search_CEOs_Hash(Q, 10)
+    for cfg in ((1, 8), (2, 9), (0, 0)):
+        g.set_search_config(*cfg)
+        gi, gp = g.search_CEOs_Hash(Q, 10)
+        _same(gi, gp, oi, op)
