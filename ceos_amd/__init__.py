@@ -10,7 +10,7 @@ reference's classes ``CEOs`` and ``streamCEOs``::
     ids, ips = index.search_coCEOs_Est(Q, k)
 
 Importing fails loudly when the native library has not been built
-(``python -m ceos_amd.build``); there is no fallback path.
+(``python ceos_amd/build.py``); there is no fallback path.
 """
 from __future__ import annotations
 
@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libceos_amd.so")
 
 if not os.path.exists(LIB_PATH):
-    raise ImportError(f"{LIB_PATH} is missing: build the HIP library with `python -m ceos_amd.build`")
+    raise ImportError(f"{LIB_PATH} is missing: build the HIP library with `python ceos_amd/build.py`")
 
 from . import CEOs  # noqa: E402  (pybind11 extension, links libceos_amd.so)
 

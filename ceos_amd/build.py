@@ -1,6 +1,6 @@
 """Builds libceos_amd.so (HIP kernels + C ABI) and the pybind11 module `CEOs` in-tree.
 
-    python -m ceos_amd.build
+    python ceos_amd/build.py
 
 Every HIP source is compiled for gfx950 only, with packed FP32 instructions
 disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
