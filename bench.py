@@ -35,6 +35,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "QPS at recall@10>=0.9, served+batched vs plain GPU; p50/p99 ms; build Mpts/s"
 HBM_PEAK_GBS = 8000.0
+# FP32 adds per second without packed FP32 instructions: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+# (the 157.3 TFLOP/s FP32 vector peak counts a packed FMA as 4 flops per lane)
+VALU_ADD_PEAK_T = 39.3
 
 
 def log(msg):
@@ -280,6 +283,19 @@ def main():
             "kernel": "k_search_fused", "kernel_ms": round(fused_ms, 4),
             "bytes_per_query": round(bytes_launch / nq), "front_kernels_ms": round(front_ms, 4)}
 
+    # ---- build roofline: the FWHT adds of every point (R repeats x 3 rounds x F log2 F) against
+    # the non-packed FP32 add peak; HBM bytes = the points read once (4d per point) plus the
+    # buckets written (2 lists x D*R columns x top_m entries x 8 B)
+    F = 1 << (w.D - 1).bit_length() if w.D >= d else 1 << (d - 1).bit_length()
+    adds_pt = w.R * 3 * F * (F.bit_length() - 1)
+    build_bytes = 4.0 * d * n + 8.0 * 2 * w.D * w.R * w.m
+    build = {"mpts_s": round(n / build_s / 1e6, 2), "ms": round(build_s * 1e3, 2),
+             "adds_per_point": adds_pt,
+             "valu": {"achieved": round(adds_pt * n / build_s / 1e12, 3), "peak": VALU_ADD_PEAK_T,
+                      "unit": "T add/s", "frac": round(adds_pt * n / build_s / 1e12 / VALU_ADD_PEAK_T, 4)},
+             "hbm": {"achieved": round(build_bytes / build_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(build_bytes / build_s / 1e9 / HBM_PEAK_GBS, 4), "bytes": round(build_bytes)}}
+
     uneven = bool(w.uneven)
     out = {
         "metric": METRIC,
@@ -309,6 +325,7 @@ def main():
                              "p99": round(float(np.percentile(lat, 99)), 4)},
         "build_mpts_s": round(n / build_s / 1e6, 2),
         "build_ms": round(build_s * 1e3, 2),
+        "build": build,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
