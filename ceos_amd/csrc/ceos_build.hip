@@ -123,8 +123,75 @@ hipError_t launch_transpose(const float* in, int64_t rows, int64_t cols, float* 
 }
 
 // ---------------------------------------------------------------------------------
+// Per column c of the column-major sample (S values): thr[2c] = the rank-th largest value
+// (close list) and thr[2c+1] = the rank-th largest negated value (far list), duplicates
+// counted. Each thread's largest value is one sample value, so the rank-th largest of the
+// kBuildNT per-thread maxima is at most the rank-th largest value (rank <= kBuildNT): every
+// value at or above it (a few hundred) is collected in LDS and the rank-th largest of
+// those is exact. Likewise for the far list with the per-thread minima. A column whose
+// collection exceeds kThrCollect falls back to a radix select over the global values.
+constexpr int kThrCollect = 2048;
+
+__device__ __forceinline__ float col_threshold(const float* __restrict__ v, int64_t S, int rank, bool far,
+                                               float bound, float* coll, SelectShared& sh, uint32_t* counter) {
+    // collect values at or beyond the bound
+    if (threadIdx.x == 0) *counter = 0;
+    __syncthreads();
+    for (int64_t i0 = 0; i0 < S; i0 += kBuildNT) {
+        const int64_t i = i0 + threadIdx.x;
+        const float x = i < S ? (far ? -v[i] : v[i]) : -INFINITY;
+        const bool take = i < S && x >= bound;
+        const uint32_t pos = wave_append(take, counter);
+        if (take && pos < (uint32_t)kThrCollect) coll[pos] = x;
+    }
+    __syncthreads();
+    const int cnt = (int)*counter;
+    uint64_t prefix;
+    int shift;
+    if (cnt <= kThrCollect) {
+        radix_select<kBuildNT>(cnt, rank, [&](int i) -> uint64_t { return (uint64_t)fkey(coll[i]) << 32; }, sh,
+                               prefix, shift, 32);
+    } else {  // rare: too many values at the bound
+        radix_select<kBuildNT>((int)S, rank,
+                               [&](int i) -> uint64_t { return (uint64_t)fkey(far ? -v[i] : v[i]) << 32; }, sh,
+                               prefix, shift, 32);
+    }
+    return (shift >= 63) ? -INFINITY : fkey_inv((uint32_t)(prefix >> 32));
+}
+
 __global__ __launch_bounds__(kBuildNT) void k_sample_thresholds(const float* __restrict__ sample, int64_t S,
                                                                 int rank, float* __restrict__ thr) {
+    __shared__ SelectShared sh;
+    __shared__ float ext[2][kBuildNT];     // per-thread maxima, negated minima
+    __shared__ float coll[kThrCollect];
+    __shared__ uint32_t counter;
+    const int col = blockIdx.x;
+    const float* v = sample + (int64_t)col * S;
+    float mx = -INFINITY, nmn = -INFINITY;  // largest value, largest negated value
+    for (int64_t i = threadIdx.x; i < S; i += kBuildNT) {
+        const float x = v[i];
+        mx = fmaxf(mx, x);
+        nmn = fmaxf(nmn, -x);
+    }
+    ext[0][threadIdx.x] = mx;
+    ext[1][threadIdx.x] = nmn;
+    __syncthreads();
+    for (int side = 0; side < 2; ++side) {
+        uint64_t prefix;
+        int shift;
+        const float* e = ext[side];
+        radix_select<kBuildNT>(kBuildNT, rank, [&](int i) -> uint64_t { return (uint64_t)fkey(e[i]) << 32; }, sh,
+                               prefix, shift, 32);
+        const float bound = (shift >= 63) ? -INFINITY : fkey_inv((uint32_t)(prefix >> 32));
+        const float t = col_threshold(v, S, rank, side == 1, bound, coll, sh, &counter);
+        if (threadIdx.x == 0) thr[2 * col + side] = t;
+        __syncthreads();
+    }
+}
+
+// Generic path (rank above kBuildNT): one workgroup per list, radix select over the column.
+__global__ __launch_bounds__(kBuildNT) void k_sample_thresholds_radix(const float* __restrict__ sample, int64_t S,
+                                                                      int rank, float* __restrict__ thr) {
     __shared__ SelectShared sh;
     const int list = blockIdx.x;
     const int col = list >> 1;
@@ -142,7 +209,10 @@ __global__ __launch_bounds__(kBuildNT) void k_sample_thresholds(const float* __r
 
 hipError_t launch_sample_thresholds(const float* sample, int64_t S, int DR, int rank, float* thr,
                                     hipStream_t st) {
-    hipLaunchKernelGGL(k_sample_thresholds, dim3(2 * DR), dim3(kBuildNT), 0, st, sample, S, rank, thr);
+    if (rank <= kBuildNT)
+        hipLaunchKernelGGL(k_sample_thresholds, dim3(DR), dim3(kBuildNT), 0, st, sample, S, rank, thr);
+    else
+        hipLaunchKernelGGL(k_sample_thresholds_radix, dim3(2 * DR), dim3(kBuildNT), 0, st, sample, S, rank, thr);
     return hipGetLastError();
 }
 

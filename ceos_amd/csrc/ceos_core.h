@@ -203,6 +203,12 @@ struct Core {
 
     // CEOs-Est projections [DR][n]
     DBuf<float> estP;
+    // build workspace, kept between builds (grow-only; released by clear()): sample
+    // projections row-major and column-major, per-list thresholds, candidate segments
+    DBuf<float> bw_rows, bw_sample, bw_thr;
+    DBuf<int64_t> bw_base;
+    DBuf<int32_t> bw_cap, bw_count;
+    DBuf<Entry> bw_ent;
     int64_t est_n = 0;
 
     std::mutex mu;
@@ -255,6 +261,8 @@ struct Core {
     void clear() {
         bkt.release(); bsize.release(); X.release(); signs.release(); dead.release();
         scr.release(); estP.release();
+        bw_rows.release(); bw_sample.release(); bw_thr.release(); bw_base.release(); bw_cap.release();
+        bw_count.release(); bw_ent.release();
         bits.clear();
         cap = head = 0; base = live = 0; est_n = 0;
         built = false;
@@ -305,26 +313,31 @@ struct Core {
         const bool exact = stride == 1;
         const int rank = exact ? m : (int)std::min<int64_t>(S, (5 * (int64_t)m * S + 2 * n - 1) / (2 * n) + 8);
 
-        DBuf<float> thr(nl);
+        DBuf<float>& thr = bw_thr;
+        thr.ensure(nl);
         {
             // project the sample row-major (coalesced), then transpose to one column per list
-            DBuf<float> rowsP((size_t)DR_ * S), sample((size_t)DR_ * S);
-            HIP_CHECK(ceos::launch_project_rows(dm, X.p, 0, stride, S, signs.p, rowsP.p, DR_, false, s_));
-            HIP_CHECK(ceos::launch_transpose(rowsP.p, S, DR_, sample.p, s_));
-            HIP_CHECK(ceos::launch_sample_thresholds(sample.p, S, DR_, rank, thr.p, s_));
-            HIP_CHECK(hipStreamSynchronize(s_));
+            bw_rows.ensure((size_t)DR_ * S);
+            bw_sample.ensure((size_t)DR_ * S);
+            HIP_CHECK(ceos::launch_project_rows(dm, X.p, 0, stride, S, signs.p, bw_rows.p, DR_, false, s_));
+            HIP_CHECK(ceos::launch_transpose(bw_rows.p, S, DR_, bw_sample.p, s_));
+            HIP_CHECK(ceos::launch_sample_thresholds(bw_sample.p, S, DR_, rank, thr.p, s_));
         }
         std::vector<int64_t> hbase(nl);
         std::vector<int32_t> hcap(nl, (int32_t)std::min<int64_t>(n, 4LL * m + 256));
         std::vector<int32_t> hcount(nl);
         std::vector<float> hthr;
-        DBuf<int64_t> dbase(nl);
-        DBuf<int32_t> dcap(nl), dcount(nl);
-        DBuf<Entry> ent;
+        DBuf<int64_t>& dbase = bw_base;
+        DBuf<int32_t>& dcap = bw_cap;
+        DBuf<int32_t>& dcount = bw_count;
+        DBuf<Entry>& ent = bw_ent;
+        dbase.ensure(nl);
+        dcap.ensure(nl);
+        dcount.ensure(nl);
         for (int iter = 0;; ++iter) {
             int64_t total = 0;
             for (int l = 0; l < nl; ++l) { hbase[l] = total; total += hcap[l]; }
-            ent.alloc((size_t)total);
+            ent.ensure((size_t)total);
             HIP_CHECK(hipMemcpyAsync(dbase.p, hbase.data(), nl * 8, hipMemcpyHostToDevice, s_));
             HIP_CHECK(hipMemcpyAsync(dcap.p, hcap.data(), nl * 4, hipMemcpyHostToDevice, s_));
             HIP_CHECK(ceos::launch_fill_i32(dcount.p, 0, nl, s_));

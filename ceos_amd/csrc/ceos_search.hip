@@ -510,8 +510,103 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
 
         // ---- b. candidates: exact top-c ----
         const int ce = hash_mode ? ncand : min(cq, ncand);
+        // Exact top-ce of nk (sum, id) keys in scratch (cand) by key: the ids go to cidl. The
+        // keys are moved to the (no longer needed) tile first when they fit.
+        auto select_from_cand = [&](int nk, bool in_tile) __attribute__((always_inline)) {
+            const bool lds_keys = in_tile || 2 * nk <= T;  // 8-byte keys fit the tile
+            if (lds_keys && !in_tile) {
+                for (int i = tid; i < nk; i += kSNT) tile64[i] = cand[i];
+                __syncthreads();
+            }
+            auto msel = [&](auto get) __attribute__((always_inline)) {
+                uint64_t prefix;
+                int shift;
+                radix_select<kSNT>(nk, ce, get, sh, prefix, shift);
+                if (tid == 0) sh.counter = 0;
+                __syncthreads();
+                compact_selected<kSNT>(nk, get, prefix, shift, &sh.counter,
+                                       [&](uint32_t pos, uint64_t k) { cidl[pos] = ckey_id(k); });
+            };
+            if (lds_keys)
+                msel([&](int i) -> uint64_t { return tile64[i]; });
+            else
+                msel([&](int i) -> uint64_t { return cand[i]; });
+            __syncthreads();
+        };
+        // Single pass with few candidates taken (ce well below the distinct ids): a lower
+        // bound L of the ce-th largest sum is read off a strided sample of kSNT tile slots
+        // (the sample value of about twice the expected rank, plus 4); every slot whose
+        // sum is at least L goes to scratch as a (sum, id) key in one pass over the tile.
+        // When at least ce keys pass, the top-ce keys are all among them and are selected
+        // exactly; otherwise (a 5-sigma event) the histogram select below runs. The tile pass
+        // writes (sum key, slot) pairs only; the ids are then fetched for the pairs kept, all
+        // loads in flight together, and the keys go to the tile (when they fit) or stay in
+        // scratch. Returns the number of keys, or -1.
+        auto sampled_compaction = [&]() __attribute__((always_inline)) -> int {
+            uint64_t* skey = reinterpret_cast<uint64_t*>(lds + lo.hist_off);  // kSNT sample keys
+            const int stepS = T / kSNT;
+            const uint32_t sb0 = keys[tid * stepS];
+            skey[tid] = sb0 != kUnset ? (uint64_t)fkey(__uint_as_float(sb0)) << 32 : 0ull;
+            if (tid == 0) { misc[14] = 0; misc[15] = 0; }
+            __syncthreads();
+            const uint64_t occm = __ballot(sb0 != kUnset);
+            if (ln == 0) atomicAdd(reinterpret_cast<uint32_t*>(&misc[14]), (uint32_t)__popcll(occm));
+            __syncthreads();
+            const int nso = misc[14];
+            const int rs = (int)((2ll * ce * nso + ncand - 1) / max(ncand, 1)) + 4;
+            if (2 * rs > nso) return -1;  // uniform: ce is a large share of the ids
+            uint64_t prefix;
+            int shift;
+            radix_select<kSNT>(kSNT, rs, [&](int i) -> uint64_t { return skey[i]; }, sh, prefix, shift, 32);
+            const uint32_t lkey = (uint32_t)(prefix >> 32);
+            for (int i0 = 0; i0 < T; i0 += kEmitU * kSNT) {  // uniform trip count (wave-level append)
+                uint32_t tm = 0;
+#pragma unroll
+                for (int u = 0; u < kEmitU; ++u) {
+                    const int i = i0 + u * kSNT + tid;
+                    const uint32_t sb = i < T ? keys[i] : kUnset;
+                    tm |= (sb != kUnset && fkey(__uint_as_float(sb)) >= lkey ? 1u : 0u) << u;
+                }
+                uint32_t pos = wave_append_n<4>((uint32_t)__popc(tm), reinterpret_cast<uint32_t*>(&misc[15]));
+#pragma unroll
+                for (int u = 0; u < kEmitU; ++u) {
+                    const int i = i0 + u * kSNT + tid;
+                    if ((tm >> u) & 1u) cand[pos++] = ((uint64_t)fkey(__uint_as_float(keys[i])) << 32) | (uint32_t)i;
+                }
+            }
+            __syncthreads();
+            const int nk = misc[15];
+            if (nk < ce) return -1;  // uniform
+            // (sum key, slot) -> (sum key, ~id); into the tile when the keys fit it
+            uint64_t* dst = 2 * nk <= T ? tile64 : cand;
+            for (int i0 = 0; i0 < nk; i0 += kEmitU * kSNT) {
+                uint64_t e[kEmitU];
+                uint32_t idv[kEmitU];
+#pragma unroll
+                for (int u = 0; u < kEmitU; ++u) {
+                    const int i = i0 + u * kSNT + tid;
+                    e[u] = i < nk ? cand[i] : 0ull;
+                }
+#pragma unroll
+                for (int u = 0; u < kEmitU; ++u) {
+                    const int i = i0 + u * kSNT + tid;
+                    idv[u] = i < nk ? keys_g[(uint32_t)e[u]] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < kEmitU; ++u) {
+                    const int i = i0 + u * kSNT + tid;
+                    if (i < nk) dst[i] = (e[u] & 0xffffffff00000000ull) | (uint64_t)(~idv[u]);
+                }
+            }
+            __syncthreads();
+            return nk;
+        };
+        int nsampled = -1;
+        if (!hash_mode && ce > 0 && P == 1 && ce < ncand) nsampled = sampled_compaction();
         if (hash_mode) {
             // no selection: every candidate is reranked
+        } else if (nsampled >= 0) {
+            select_from_cand(nsampled, 2 * nsampled <= T);
         } else if (ce > 0 && P == 1) {
             // Single pass: the tile holds each occupied slot's sum (kUnset: empty) and
             // keys_g its id. Keys are (sum desc, id asc). A histogram of the sums over
@@ -631,25 +726,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             }
             __syncthreads();
         } else if (ce > 0) {
-            const bool lds_keys = 2 * ncand <= T;  // 8-byte keys fit the (now free) tile
-            if (lds_keys) {
-                for (int i = tid; i < ncand; i += kSNT) tile64[i] = cand[i];
-                __syncthreads();
-            }
-            auto msel = [&](auto get) __attribute__((always_inline)) {
-                uint64_t prefix;
-                int shift;
-                radix_select<kSNT>(ncand, ce, get, sh, prefix, shift);
-                if (tid == 0) sh.counter = 0;
-                __syncthreads();
-                compact_selected<kSNT>(ncand, get, prefix, shift, &sh.counter,
-                                       [&](uint32_t pos, uint64_t k) { cidl[pos] = ckey_id(k); });
-            };
-            if (lds_keys)
-                msel([&](int i) -> uint64_t { return tile64[i]; });
-            else
-                msel([&](int i) -> uint64_t { return cand[i]; });
-            __syncthreads();
+            select_from_cand(ncand, false);
         }
 
         // ---- c. rerank ----
