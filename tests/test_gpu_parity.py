@@ -117,3 +117,32 @@ def test_ceos_est_small(CEOs, oracle):
     gi, gp = g.search(Q, 10)
     oi, op = o.search(Q, 10)
     _compare_search(gi, gp, oi, op, tol_frac=0.0)
+
+
+def test_buckets_sampled_thresholds(CEOs, oracle):
+    """150,000 points: the build samples every 4th point for its per-list thresholds
+    (rank about 90), so the exact per-list selection runs on the candidates that pass
+    them. Buckets must equal the oracle's, which scans every point."""
+    X = _rand(150_000, 24, 11)
+    g, o = _build_pair(CEOs, oracle, X, 64, 2, 120)
+    _compare_buckets(g, o)
+
+
+def test_buckets_sampled_thresholds_duplicates(CEOs, oracle):
+    """A third of the points are one repeated row, so in every column tens of thousands of
+    sample values tie at the threshold bound: the threshold kernel's collected tail
+    overflows and it falls back to a radix select over the whole column. Buckets must
+    still equal the oracle's (ties compared as sets)."""
+    X = _rand(120_000, 16, 12)
+    X[::3] = X[5]
+    g, o = _build_pair(CEOs, oracle, X, 32, 2, 100)
+    _compare_buckets(g, o)
+
+
+def test_buckets_large_top_m_exact_sample(CEOs, oracle):
+    """n below the sample size: every point is sampled and the threshold rank equals
+    top_m (300, above the per-column kernel's 256 per-thread bounds), which takes the
+    one-list-per-workgroup radix path."""
+    X = _rand(6000, 40, 13)
+    g, o = _build_pair(CEOs, oracle, X, 64, 2, 300)
+    _compare_buckets(g, o)
