@@ -117,11 +117,46 @@ struct Proj<L, true> {
         wave_sync();
     }
 
+    // Relayout from phase bp to the last phase (b = L - e = 6) in registers, for bp >= 4:
+    // in layout bp, lane bit j in [bp, 6) holds element bit j + e and register bit j - bp
+    // holds element bit j; in layout 6 those trade places. v_permlane16_swap exchanges
+    // rows 1 and 3 of its first operand with rows 0 and 2 of its second (lane bit 4 against
+    // the register pair's bit), v_permlane32_swap the upper half of the first with the lower
+    // half of the second (lane bit 5). Afterwards register bit i < s = 6 - bp holds element
+    // bit bp + e + i and register bit i >= s element bit bp + i, so the register index is
+    // rotated right by s bits (a renaming, no instructions).
+    template <int bp>
+    __device__ static __forceinline__ void relayout_swap(float (&y)[E]) {
+        constexpr int s = (L - e) - bp;
+#pragma unroll
+        for (int j = bp; j < L - e; ++j) {
+            const int hb = 1 << (j - bp);
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                if (!(r & hb)) {
+                    const auto t = j == 4
+                        ? __builtin_amdgcn_permlane16_swap(__float_as_uint(y[r]), __float_as_uint(y[r + hb]), false, false)
+                        : __builtin_amdgcn_permlane32_swap(__float_as_uint(y[r]), __float_as_uint(y[r + hb]), false, false);
+                    y[r] = __uint_as_float(t[0]);
+                    y[r + hb] = __uint_as_float(t[1]);
+                }
+            }
+        }
+        float t[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) t[(r >> s) | ((r & ((1 << s) - 1)) << (e - s))] = y[r];
+#pragma unroll
+        for (int r = 0; r < E; ++r) y[r] = t[r];
+    }
+
     template <int covered, int bprev>
     struct Phase {
         static constexpr int b = (covered + e <= L) ? covered : (L - e);
         __device__ static __forceinline__ void run(float (&y)[E], float* lds, int ln) {
-            if (covered > 0) relayout(y, lds, ln, bprev, b);
+            if constexpr (covered > 0 && b == L - e && bprev >= 4)
+                relayout_swap<bprev>(y);
+            else if (covered > 0)
+                relayout(y, lds, ln, bprev, b);
             stages<b, covered, b + e>(y);
             Phase<b + e, b>::run(y, lds, ln);
         }
