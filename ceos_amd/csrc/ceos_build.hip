@@ -126,45 +126,20 @@ hipError_t launch_transpose(const float* in, int64_t rows, int64_t cols, float* 
 // Per column c of the column-major sample (S values): thr[2c] = the rank-th largest value
 // (close list) and thr[2c+1] = the rank-th largest negated value (far list), duplicates
 // counted. Each thread's largest value is one sample value, so the rank-th largest of the
-// kBuildNT per-thread maxima is at most the rank-th largest value (rank <= kBuildNT): every
-// value at or above it (a few hundred) is collected in LDS and the rank-th largest of
-// those is exact. Likewise for the far list with the per-thread minima. A column whose
-// collection exceeds kThrCollect falls back to a radix select over the global values.
+// kBuildNT per-thread maxima is at most the rank-th largest value (rank <= kBuildNT);
+// likewise for the negated values with the per-thread minima. A second read of the column
+// collects every value at or above the first bound and every negated value at or above the
+// second (a few hundred each) in LDS, and the rank-th largest of each collection is exact.
+// A side whose collection exceeds kThrCollect falls back to a radix select over the column.
 constexpr int kThrCollect = 2048;
-
-__device__ __forceinline__ float col_threshold(const float* __restrict__ v, int64_t S, int rank, bool far,
-                                               float bound, float* coll, SelectShared& sh, uint32_t* counter) {
-    // collect values at or beyond the bound
-    if (threadIdx.x == 0) *counter = 0;
-    __syncthreads();
-    for (int64_t i0 = 0; i0 < S; i0 += kBuildNT) {
-        const int64_t i = i0 + threadIdx.x;
-        const float x = i < S ? (far ? -v[i] : v[i]) : -INFINITY;
-        const bool take = i < S && x >= bound;
-        const uint32_t pos = wave_append(take, counter);
-        if (take && pos < (uint32_t)kThrCollect) coll[pos] = x;
-    }
-    __syncthreads();
-    const int cnt = (int)*counter;
-    uint64_t prefix;
-    int shift;
-    if (cnt <= kThrCollect) {
-        radix_select<kBuildNT>(cnt, rank, [&](int i) -> uint64_t { return (uint64_t)fkey(coll[i]) << 32; }, sh,
-                               prefix, shift, 32);
-    } else {  // rare: too many values at the bound
-        radix_select<kBuildNT>((int)S, rank,
-                               [&](int i) -> uint64_t { return (uint64_t)fkey(far ? -v[i] : v[i]) << 32; }, sh,
-                               prefix, shift, 32);
-    }
-    return (shift >= 63) ? -INFINITY : fkey_inv((uint32_t)(prefix >> 32));
-}
 
 __global__ __launch_bounds__(kBuildNT) void k_sample_thresholds(const float* __restrict__ sample, int64_t S,
                                                                 int rank, float* __restrict__ thr) {
     __shared__ SelectShared sh;
-    __shared__ float ext[2][kBuildNT];     // per-thread maxima, negated minima
-    __shared__ float coll[kThrCollect];
-    __shared__ uint32_t counter;
+    __shared__ float ext[2][kBuildNT];       // per-thread maxima, negated minima
+    __shared__ float coll[2][kThrCollect];   // collected values, negated values
+    __shared__ uint32_t counter[2];
+    __shared__ float bound[2];
     const int col = blockIdx.x;
     const float* v = sample + (int64_t)col * S;
     float mx = -INFINITY, nmn = -INFINITY;  // largest value, largest negated value
@@ -175,6 +150,7 @@ __global__ __launch_bounds__(kBuildNT) void k_sample_thresholds(const float* __r
     }
     ext[0][threadIdx.x] = mx;
     ext[1][threadIdx.x] = nmn;
+    if (threadIdx.x == 0) counter[0] = counter[1] = 0;
     __syncthreads();
     for (int side = 0; side < 2; ++side) {
         uint64_t prefix;
@@ -182,10 +158,36 @@ __global__ __launch_bounds__(kBuildNT) void k_sample_thresholds(const float* __r
         const float* e = ext[side];
         radix_select<kBuildNT>(kBuildNT, rank, [&](int i) -> uint64_t { return (uint64_t)fkey(e[i]) << 32; }, sh,
                                prefix, shift, 32);
-        const float bound = (shift >= 63) ? -INFINITY : fkey_inv((uint32_t)(prefix >> 32));
-        const float t = col_threshold(v, S, rank, side == 1, bound, coll, sh, &counter);
-        if (threadIdx.x == 0) thr[2 * col + side] = t;
-        __syncthreads();
+        if (threadIdx.x == 0) bound[side] = (shift >= 63) ? -INFINITY : fkey_inv((uint32_t)(prefix >> 32));
+    }
+    __syncthreads();
+    const float b0 = bound[0], b1 = bound[1];
+    for (int64_t i0 = 0; i0 < S; i0 += kBuildNT) {  // uniform trip count (wave-level appends)
+        const int64_t i = i0 + threadIdx.x;
+        const float x = i < S ? v[i] : 0.0f;
+        const bool t0 = i < S && x >= b0;
+        const bool t1 = i < S && -x >= b1;
+        const uint32_t p0 = wave_append(t0, &counter[0]);
+        const uint32_t p1 = wave_append(t1, &counter[1]);
+        if (t0 && p0 < (uint32_t)kThrCollect) coll[0][p0] = x;
+        if (t1 && p1 < (uint32_t)kThrCollect) coll[1][p1] = -x;
+    }
+    __syncthreads();
+    for (int side = 0; side < 2; ++side) {
+        const int cnt = (int)counter[side];
+        const float* c = coll[side];
+        const bool neg = side == 1;
+        uint64_t prefix;
+        int shift;
+        if (cnt <= kThrCollect) {
+            radix_select<kBuildNT>(cnt, rank, [&](int i) -> uint64_t { return (uint64_t)fkey(c[i]) << 32; }, sh,
+                                   prefix, shift, 32);
+        } else {  // rare: too many values at the bound
+            radix_select<kBuildNT>((int)S, rank,
+                                   [&](int i) -> uint64_t { return (uint64_t)fkey(neg ? -v[i] : v[i]) << 32; }, sh,
+                                   prefix, shift, 32);
+        }
+        if (threadIdx.x == 0) thr[2 * col + side] = (shift >= 63) ? -INFINITY : fkey_inv((uint32_t)(prefix >> 32));
     }
 }
 
