@@ -146,3 +146,23 @@ def test_buckets_large_top_m_exact_sample(CEOs, oracle):
     X = _rand(6000, 40, 13)
     g, o = _build_pair(CEOs, oracle, X, 64, 2, 300)
     _compare_buckets(g, o)
+
+
+@pytest.mark.parametrize("n_cand", [200, 1000, 5000])
+def test_search_candidate_select_paths(CEOs, oracle, n_cand):
+    """About 12,700 distinct candidates per query in one tile pass. n_cand 200 and 1000
+    take the sampled-bound compaction (a lower bound of the n_cand-th score from 512
+    sampled tile slots, then an exact select among the sums above it); 5000 takes the
+    value-histogram select. Every query's top-10 must equal the oracle's."""
+    rng = np.random.default_rng(21)
+    n, d = 200_000, 32
+    X = (rng.standard_normal((n, d)) * rng.lognormal(0, 0.5, (n, 1))).astype(np.float32)
+    Q = rng.standard_normal((300, d)).astype(np.float32)
+    Q /= np.linalg.norm(Q, axis=1, keepdims=True)
+    g, o = _build_pair(CEOs, oracle, X, 256, 2, 2000)
+    for obj in (g, o):
+        obj.n_probed_vectors = 10
+        obj.n_cand = n_cand
+    gi, gp = g.search_coCEOs_Est(Q, 10)
+    oi, op = o.search_coCEOs_Est(Q, 10)
+    _compare_search(gi, gp, oi, op, tol_frac=0.0)
