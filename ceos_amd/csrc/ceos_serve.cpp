@@ -314,6 +314,14 @@ int make_server(Core& x, ceos_index* hi, ceos_stream* hs, const ceos_server_conf
             if (rc != CEOS_OK) throw DevError(g_err);
             HIP_CHECK(hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking));
             HIP_CHECK(hipEventCreateWithFlags(&sl->ev, hipEventDisableTiming));
+            {
+                // capture the slot's graph now and replay it once with an empty batch, so the
+                // first requests do not pay the capture and the graph upload
+                std::lock_guard<std::mutex> g(x.mu);
+                sl->plan->capture(0, 0);
+                sl->plan->launch(0, sl->st);
+                HIP_CHECK(hipStreamSynchronize(sl->st));
+            }
             sl->hQ.alloc((size_t)cfg->max_batch * x.d);
             sl->hk.alloc(cfg->max_batch);
             sl->hs.alloc(cfg->max_batch);
