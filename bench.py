@@ -184,6 +184,16 @@ def main():
     log(f"kernel events: fused search {fused_ms:.3f} ms/launch, projection+probe select {front_ms:.3f} ms/launch")
     index.set_search_config(0, 0)
 
+    # ---- the practical streaming-read rate of this GPU: a plain grid-stride 16-byte read
+    # over 4 GiB (8192 workgroups of 256), timed with HIP events on the same stream
+    xr = torch.ones(1 << 30, dtype=torch.float32, device=dev)
+    orr = torch.empty(8192 * 256, dtype=torch.float32, device=dev)
+    read_gbs = ctypes.c_double()
+    capi.check(L.ceos_read_bandwidth(xr.data_ptr(), xr.numel(), orr.data_ptr(), orr.numel(), 10, stream,
+                                     ctypes.byref(read_gbs)))
+    del xr, orr
+    log(f"streaming read of 4 GiB: {read_gbs.value:,.0f} GB/s")
+
     if args.copy_mib:
         # a plain copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE and
         # WRITE_SIZE give the scale of the counters against known bytes
@@ -281,7 +291,9 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes_per_launch": round(bytes_launch),
             "kernel": "k_search_fused", "kernel_ms": round(fused_ms, 4),
-            "bytes_per_query": round(bytes_launch / nq), "front_kernels_ms": round(front_ms, 4)}
+            "bytes_per_query": round(bytes_launch / nq), "front_kernels_ms": round(front_ms, 4),
+            "stream_read_gbs": round(read_gbs.value, 1),
+            "frac_of_stream_read": round(achieved / read_gbs.value, 4)}
 
     # ---- build roofline: the FWHT adds of every point (R repeats x 3 rounds x F log2 F) against
     # the non-packed FP32 add peak; HBM bytes = the points read once (4d per point) plus the

@@ -53,6 +53,7 @@ def lib():
             "ceos_plan_config": (i32, [vp, P, P, P]),
             "ceos_set_kernel_timing": (i32, [vp, i32]),
             "ceos_get_kernel_timing": (i32, [vp, P, P, P]),
+            "ceos_read_bandwidth": (i32, [P, i64, P, i64, i32, vp, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

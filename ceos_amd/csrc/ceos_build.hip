@@ -476,6 +476,29 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
         p[i] = v;
 }
 
+// Streaming-read calibration: every thread sums 16-byte loads over a grid-stride loop
+// (four in flight) and writes one float. n4 = number of float4 elements.
+__global__ __launch_bounds__(256) void k_stream_read(const float4* __restrict__ x, int64_t n4,
+                                                     float* __restrict__ out) {
+    float s = 0.0f;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const float4 a = x[i], b = x[i + stride], c = x[i + 2 * stride], d = x[i + 3 * stride];
+        s += (a.x + a.y + a.z + a.w) + (b.x + b.y + b.z + b.w) + (c.x + c.y + c.z + c.w) + (d.x + d.y + d.z + d.w);
+    }
+    for (; i < n4; i += stride) {
+        const float4 a = x[i];
+        s += a.x + a.y + a.z + a.w;
+    }
+    out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+hipError_t launch_stream_read(const float* x, int64_t n, float* out, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_stream_read, dim3(grid), dim3(256), 0, st, reinterpret_cast<const float4*>(x), n / 4, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     const int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);

@@ -355,6 +355,34 @@ int ceos_get_search_config(const ceos_index* h, int* wg_per_cu, int* table_log2,
     });
 }
 
+int ceos_read_bandwidth(const float* x, int64_t n, float* out, int64_t n_out, int iters, void* stream,
+                        double* gbs) {
+    return guarded([&] {
+        if (!x || !out || !gbs) throw ArgError("null argument");
+        if (n <= 0 || n % 4 != 0 || (reinterpret_cast<uintptr_t>(x) & 15) != 0)
+            throw ArgError("n must be a positive multiple of 4 and x 16-byte aligned");
+        if (n_out < 256 || n_out % 256 != 0 || n_out / 256 > (int64_t)1 << 20)
+            throw ArgError("n_out must be a multiple of 256 with at most 2^20 workgroups");
+        if (iters <= 0) throw ArgError("iters must be positive");
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        const int grid = (int)(n_out / 256);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIP_CHECK(hipEventCreate(&e0));
+        hipError_t err = hipEventCreate(&e1);
+        float ms = 0.0f;
+        if (err == hipSuccess) err = ceos::launch_stream_read(x, n, out, grid, st);  // warm
+        if (err == hipSuccess) err = hipEventRecord(e0, st);
+        for (int t = 0; t < iters && err == hipSuccess; ++t) err = ceos::launch_stream_read(x, n, out, grid, st);
+        if (err == hipSuccess) err = hipEventRecord(e1, st);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        HIP_CHECK(err);
+        *gbs = (double)n * 4.0 * iters / ((double)ms * 1e6);
+    });
+}
+
 int ceos_set_kernel_timing(ceos_index* h, int on) {
     return guarded([&] {
         if (!h) throw ArgError("null handle");

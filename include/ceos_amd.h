@@ -127,6 +127,14 @@ int ceos_get_search_config(const ceos_index* h, int* wg_per_cu, int* table_log2,
 int ceos_set_kernel_timing(ceos_index* h, int on);
 int ceos_get_kernel_timing(ceos_index* h, double* front_ms, double* fused_ms, int* launches);
 
+/* HBM read calibration (no reference counterpart; for benchmarks): times `iters`
+ * launches of a plain streaming-read kernel over the device array x (n floats, n a
+ * multiple of 4, 16-byte aligned) on `stream`, with n_out/256 workgroups of 256 threads
+ * writing one partial sum each into out (n_out floats, a multiple of 256), and returns
+ * the read rate in GB/s. Allocates nothing. */
+int ceos_read_bandwidth(const float* x, int64_t n, float* out, int64_t n_out, int iters, void* stream,
+                        double* gbs);
+
 /* hipStream_t the index launches on (as void*). */
 void* ceos_get_stream(const ceos_index* h);
 

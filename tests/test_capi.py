@@ -67,3 +67,16 @@ def test_module_errors_raise_instead_of_exit(CEOs):
     s = CEOs.streamCEOs(8)
     with pytest.raises(RuntimeError):                                     # not built
         s.hash_search(np.zeros((3, 8), np.float32), 5)
+
+
+def test_read_bandwidth_argument_checks():
+    """The calibration entry point rejects bad arguments before touching a device."""
+    L = capi.lib()
+    g = C.c_double()
+    buf = np.zeros(1024, np.float32)
+    p = buf.ctypes.data
+    assert L.ceos_read_bandwidth(None, 1024, p, 256, 1, None, C.byref(g)) == 1      # null array
+    assert L.ceos_read_bandwidth(p, 1022, p, 256, 1, None, C.byref(g)) == 1         # n not a multiple of 4
+    assert L.ceos_read_bandwidth(p, 1024, p, 100, 1, None, C.byref(g)) == 1         # n_out not a multiple of 256
+    assert L.ceos_read_bandwidth(p, 1024, p, 256, 0, None, C.byref(g)) == 1         # iters
+    assert b"iters" in L.ceos_last_error()
