@@ -251,13 +251,20 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         auto phase1_chunk = [&](const Entry (&buf)[CH][EPT], int j0, uint32_t pass) {
             uint32_t id[B], slot[B], b1[B], b2[B], hh[B];
             float v[B];
+            // the chunk's probe lengths (0 past the last probe), read once
+            int limc[CH];
+#pragma unroll
+            for (int jj = 0; jj < CH; ++jj) limc[jj] = j0 + jj < sq ? lims[j0 + jj] : 0;
+            uint32_t inpos = 0;  // bit b: entry b is a position of a probed list
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                const int j = j0 + b / EPT, t = tid + (b % EPT) * kSNT;
+                const int t = tid + (b % EPT) * kSNT;
                 id[b] = buf[b / EPT][b % EPT].x - rows.base;
                 v[b] = __uint_as_float(buf[b / EPT][b % EPT].y);
                 hh[b] = idhash(id[b]);
-                const bool valid = j < sq && t < lims[min(j, jlast)] && (P == 1 || pass_of(hh[b], P) == pass);
+                const bool pos_ok = t < limc[b / EPT];
+                inpos |= (pos_ok ? 1u : 0u) << b;
+                const bool valid = pos_ok && (P == 1 || pass_of(hh[b], P) == pass);
                 slot[b] = valid ? kToPlace : kEmptyKey;  // kEmptyKey: not in this pass
                 b1[b] = bucket1(hh[b], lognbk);
                 b2[b] = bucket2(hh[b], b1[b], lognbk);
@@ -318,7 +325,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     else
                         misc[4] = 1;  // stash list full: the pass restarts with more partitions
                 }
-                if (!hash_mode && j < sq && t < lims[min(j, jlast)])
+                if (!hash_mode && ((inpos >> b) & 1u))
                     pairs[(int64_t)j * mp + t] = make_uint2(slot[b] == kStashMark ? kEmptyKey : slot[b],
                                                             __float_as_uint(v[b]));
             }
