@@ -3,7 +3,10 @@
 // searches a whole query matrix per call (src/CEOs.cpp:450-694).
 //
 // One worker thread owns the device side. Requests (one query each, with its own k,
-// n_probed_vectors and n_cand) wait in an arrival-ordered queue. Whenever one of the
+// n_probed_vectors and n_cand) come in submissions: a submission holds its rows,
+// parameters and arrival offsets, prepared outside the server lock and queued in one
+// step, and the arrival clock starts when it is queued. The worker admits due requests
+// from each submission's cursor, earliest arrival first. Whenever one of the
 // server's `inflight` batch slots is free, the worker takes every request that has
 // arrived (up to max_batch) into that slot: pinned host staging -> device buffers of a
 // captured search plan -> plan replay -> results back, all on the slot's own stream,
@@ -26,15 +29,28 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-struct Request {
-    std::vector<float> q;
-    int k = 0, s = 0, c = 0;
-    Clock::time_point arrival;
-    bool done = false;
-    bool failed = false;
-    std::vector<int32_t> ids;
-    std::vector<float> ips;
-    double latency_ms = 0.0;
+// One submission: n requests with tickets first_ticket .. first_ticket + n - 1.
+struct Feed {
+    int64_t first_ticket = 0;
+    int n = 0;
+    std::vector<float> Q;              // n x d
+    std::vector<int32_t> k, s, c;
+    std::vector<double> off;           // arrival offsets (seconds after t0)
+    std::vector<int32_t> order;        // request indices by ascending offset (admission order)
+    int next = 0;                      // admitted so far (position in order)
+    Clock::time_point t0;
+    enum : uint8_t { kQueued = 0, kDone = 1, kFailed = 2, kTaken = 3 };
+    std::vector<uint8_t> state;
+    std::vector<std::vector<int32_t>> ids;
+    std::vector<std::vector<float>> ips;
+    std::vector<double> latency_ms;
+    int untaken = 0;                   // results not yet returned by ceos_server_result
+
+    Clock::time_point arrival(int i) const {
+        return t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(off[i]));
+    }
+    bool pending() const { return next < n; }
+    Clock::time_point next_arrival() const { return arrival(order[next]); }
 };
 
 struct Update {
@@ -84,11 +100,12 @@ struct ceos_server {
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    // (arrival, ticket), earliest first
-    std::priority_queue<std::pair<Clock::time_point, int64_t>, std::vector<std::pair<Clock::time_point, int64_t>>,
+    std::deque<std::unique_ptr<Feed>> feeds;  // by first ticket
+    // (next arrival, submission) for every submission with requests not yet admitted
+    std::priority_queue<std::pair<Clock::time_point, Feed*>, std::vector<std::pair<Clock::time_point, Feed*>>,
                         std::greater<>>
-        pending;
-    std::unordered_map<int64_t, Request> reqs;
+        due;
+    int64_t queued = 0;                       // requests submitted and not yet admitted
     std::deque<Update> updates;
     int64_t next_ticket = 1;
     bool stop = false;
@@ -118,18 +135,54 @@ struct ceos_server {
 
     int k_max() const { return cfg.k_max; }
 
+    // The submission holding a ticket and the request's index in it, or null (mu held).
+    Feed* find(int64_t ticket, int* idx) {
+        auto it = std::upper_bound(feeds.begin(), feeds.end(), ticket,
+                                   [](int64_t t, const std::unique_ptr<Feed>& f) { return t < f->first_ticket; });
+        if (it == feeds.begin()) return nullptr;
+        Feed* f = (--it)->get();
+        if (ticket >= f->first_ticket + f->n) return nullptr;
+        *idx = (int)(ticket - f->first_ticket);
+        return f;
+    }
+    // The submission whose next request arrives first, or null when none is queued (mu held).
+    Feed* earliest() { return due.empty() ? nullptr : due.top().second; }
+    // Admits the next request of the earliest submission: its index (mu held).
+    int admit() {
+        Feed* f = due.top().second;
+        due.pop();
+        const int i = f->order[f->next++];
+        if (f->pending())
+            due.push({f->next_arrival(), f});
+        else {  // every row admitted: drop the rows
+            f->Q.clear();
+            f->Q.shrink_to_fit();
+        }
+        --queued;
+        return i;
+    }
+    void release_if_taken(Feed* f) {
+        if (f->untaken > 0 || f->pending()) return;
+        int idx = 0;
+        if (find(f->first_ticket, &idx) != f) return;
+        auto it = std::upper_bound(feeds.begin(), feeds.end(), f->first_ticket,
+                                   [](int64_t t, const std::unique_ptr<Feed>& g) { return t < g->first_ticket; });
+        feeds.erase(--it);
+    }
+
     // Moves a finished slot's results to its requests (mu held).
     void reap(Slot& sl) {
         const auto now = Clock::now();
         for (size_t i = 0; i < sl.tickets.size(); ++i) {
-            auto it = reqs.find(sl.tickets[i]);
-            if (it == reqs.end()) continue;
-            Request& r = it->second;
-            r.ids.assign(sl.hI.p + i * cfg.k_max, sl.hI.p + i * cfg.k_max + r.k);
-            r.ips.assign(sl.hP.p + i * cfg.k_max, sl.hP.p + i * cfg.k_max + r.k);
-            r.latency_ms = std::chrono::duration<double, std::milli>(now - r.arrival).count();
-            r.done = true;
-            lat_ms.push_back(r.latency_ms);
+            int idx = 0;
+            Feed* f = find(sl.tickets[i], &idx);
+            if (!f) continue;
+            const int kq = f->k[idx];
+            f->ids[idx].assign(sl.hI.p + i * cfg.k_max, sl.hI.p + i * cfg.k_max + kq);
+            f->ips[idx].assign(sl.hP.p + i * cfg.k_max, sl.hP.p + i * cfg.k_max + kq);
+            f->latency_ms[idx] = std::chrono::duration<double, std::milli>(now - f->arrival(idx)).count();
+            f->state[idx] = Feed::kDone;
+            lat_ms.push_back(f->latency_ms[idx]);
             ++completed;
         }
         last_done = now;
@@ -140,12 +193,13 @@ struct ceos_server {
 
     void fail_all(const std::string& msg) {  // mu held; the server stops serving
         worker_error = msg;
-        while (!pending.empty()) pending.pop();
-        for (auto& kv : reqs)
-            if (!kv.second.done) {
-                kv.second.done = true;
-                kv.second.failed = true;
-            }
+        for (auto& f : feeds) {
+            for (int i = 0; i < f->n; ++i)
+                if (f->state[i] == Feed::kQueued) f->state[i] = Feed::kFailed;
+            f->next = f->n;
+        }
+        queued = 0;
+        while (!due.empty()) due.pop();
         cv_done.notify_all();
     }
 
@@ -155,18 +209,17 @@ struct ceos_server {
         const auto now = Clock::now();
         const int d = core->d;
         int nb = 0;
-        while (!pending.empty() && nb < cfg.max_batch && pending.top().first <= now) {
-            const int64_t t = pending.top().second;
-            pending.pop();
-            Request& r = reqs.at(t);
-            std::memcpy(sl.hQ.p + (size_t)nb * d, r.q.data(), (size_t)d * 4);
-            sl.hk.p[nb] = r.k;
-            sl.hs.p[nb] = r.s;
-            sl.hc.p[nb] = r.c;
-            r.q.clear();
-            r.q.shrink_to_fit();
-            sl.tickets.push_back(t);
+        while (nb < cfg.max_batch) {
+            Feed* f = earliest();
+            if (!f || f->next_arrival() > now) break;
+            const int i = f->order[f->next];
+            std::memcpy(sl.hQ.p + (size_t)nb * d, f->Q.data() + (size_t)i * d, (size_t)d * 4);
+            sl.hk.p[nb] = f->k[i];
+            sl.hs.p[nb] = f->s[i];
+            sl.hc.p[nb] = f->c[i];
+            sl.tickets.push_back(f->first_ticket + i);
             ++nb;
+            admit();
         }
         if (nb == 0) return;
         sl.busy = true;
@@ -227,7 +280,7 @@ struct ceos_server {
             }
             bool any_busy = false;
             for (auto& sl : slots) any_busy |= sl->busy;
-            if (stop && pending.empty() && updates.empty() && !any_busy) break;
+            if (stop && queued == 0 && updates.empty() && !any_busy) break;
             // updates: between batches
             if (!updates.empty()) {
                 if (any_busy) {
@@ -248,7 +301,8 @@ struct ceos_server {
             }
             // launch into a free slot when a request has arrived
             const auto now = Clock::now();
-            const bool ready = !pending.empty() && pending.top().first <= now;
+            Feed* nf = earliest();
+            const bool ready = nf && nf->next_arrival() <= now;
             Slot* free_slot = nullptr;
             for (auto& sl : slots)
                 if (!sl->busy) {
@@ -262,10 +316,10 @@ struct ceos_server {
             // wait: for a batch to finish (poll), an arrival, or a submission
             if (any_busy) {
                 cv_work.wait_for(lk, std::chrono::microseconds(20));
-            } else if (!pending.empty()) {
-                cv_work.wait_until(lk, pending.top().first);
+            } else if (nf) {
+                cv_work.wait_until(lk, nf->next_arrival());
             } else {
-                cv_work.wait(lk, [&] { return stop || !pending.empty() || !updates.empty(); });
+                cv_work.wait(lk, [&] { return stop || queued > 0 || !updates.empty(); });
             }
         }
     }
@@ -378,29 +432,40 @@ int ceos_server_submit(ceos_server* s, const float* Q, int n, int d, const int32
         if (d != s->core->d) throw ArgError("feature dimension does not match the index");
         if (n <= 0) throw ArgError("n must be positive");
         for (int i = 0; i < n; ++i) check_req(s, k[i], sv[i], c[i]);
-        const auto t0 = Clock::now();
+        // the submission is prepared outside the lock; its clock starts when it is queued
+        auto f = std::make_unique<Feed>();
+        f->n = n;
+        f->Q.assign(Q, Q + (size_t)n * d);
+        f->k.assign(k, k + n);
+        f->s.assign(sv, sv + n);
+        f->c.assign(c, c + n);
+        f->off.assign(n, 0.0);
+        if (arrival_s)
+            for (int i = 0; i < n; ++i) f->off[i] = std::max(0.0, arrival_s[i]);
+        f->order.resize(n);
+        for (int i = 0; i < n; ++i) f->order[i] = i;
+        std::stable_sort(f->order.begin(), f->order.end(), [&](int32_t a, int32_t b) { return f->off[a] < f->off[b]; });
+        f->state.assign(n, Feed::kQueued);
+        f->ids.resize(n);
+        f->ips.resize(n);
+        f->latency_ms.assign(n, 0.0);
+        f->untaken = n;
         {
             std::lock_guard<std::mutex> g(s->mu);
             if (!s->worker_error.empty()) throw DevError("server stopped: " + s->worker_error);
-            if (s->cfg.max_pending > 0 && (int64_t)s->pending.size() + n > s->cfg.max_pending)
+            if (s->cfg.max_pending > 0 && s->queued + n > s->cfg.max_pending)
                 throw MemError("server queue is full");
-            *first_ticket = s->next_ticket;
-            for (int i = 0; i < n; ++i) {
-                Request r;
-                r.q.assign(Q + (size_t)i * d, Q + (size_t)(i + 1) * d);
-                r.k = k[i];
-                r.s = sv[i];
-                r.c = c[i];
-                r.arrival = arrival_s ? t0 + std::chrono::duration_cast<Clock::duration>(
-                                                 std::chrono::duration<double>(arrival_s[i]))
-                                      : t0;
-                if (!s->any_arrival || r.arrival < s->first_arrival) s->first_arrival = r.arrival;
-                s->any_arrival = true;
-                const int64_t t = s->next_ticket++;
-                s->pending.push({r.arrival, t});
-                s->reqs.emplace(t, std::move(r));
-            }
+            f->first_ticket = s->next_ticket;
+            s->next_ticket += n;
+            *first_ticket = f->first_ticket;
+            f->t0 = Clock::now();
+            const auto first = f->arrival(f->order[0]);
+            if (!s->any_arrival || first < s->first_arrival) s->first_arrival = first;
+            s->any_arrival = true;
+            s->queued += n;
             s->submitted += n;
+            s->due.push({first, f.get()});
+            s->feeds.push_back(std::move(f));
         }
         s->cv_work.notify_one();
     });
@@ -413,29 +478,35 @@ int ceos_server_result(ceos_server* s, int64_t ticket, int wait, int32_t* ids, f
         return CEOS_EINVAL;
     }
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end()) {
+    int idx = 0;
+    Feed* f = s->find(ticket, &idx);
+    if (!f || f->state[idx] == Feed::kTaken) {
         g_err = "unknown ticket " + std::to_string(ticket);
         return CEOS_EINVAL;
     }
-    if (!it->second.done) {
+    if (f->state[idx] == Feed::kQueued) {
         if (!wait) {
             g_err = "not finished";
             return CEOS_EAGAIN;
         }
-        s->cv_done.wait(lk, [&] { return it->second.done; });
+        s->cv_done.wait(lk, [&] { return f->state[idx] != Feed::kQueued; });
     }
-    Request& r = it->second;
-    if (r.failed) {
+    const bool failed = f->state[idx] == Feed::kFailed;
+    if (!failed) {
+        if (ids) std::copy(f->ids[idx].begin(), f->ids[idx].end(), ids);
+        if (ips) std::copy(f->ips[idx].begin(), f->ips[idx].end(), ips);
+        if (k_out) *k_out = f->k[idx];
+        if (latency_ms) *latency_ms = f->latency_ms[idx];
+    }
+    f->state[idx] = Feed::kTaken;
+    f->ids[idx] = std::vector<int32_t>();
+    f->ips[idx] = std::vector<float>();
+    --f->untaken;
+    s->release_if_taken(f);
+    if (failed) {
         g_err = s->worker_error;
-        s->reqs.erase(it);
         return CEOS_EDEVICE;
     }
-    if (ids) std::copy(r.ids.begin(), r.ids.end(), ids);
-    if (ips) std::copy(r.ips.begin(), r.ips.end(), ips);
-    if (k_out) *k_out = r.k;
-    if (latency_ms) *latency_ms = r.latency_ms;
-    s->reqs.erase(it);
     g_err.clear();
     return CEOS_OK;
 }
@@ -480,7 +551,7 @@ int ceos_server_drain(ceos_server* s) {
         std::unique_lock<std::mutex> lk(s->mu);
         s->cv_done.wait(lk, [&] {
             if (!s->worker_error.empty()) return true;
-            if (!s->pending.empty() || !s->updates.empty()) return false;
+            if (s->queued > 0 || !s->updates.empty()) return false;
             for (auto& sl : s->slots)
                 if (sl->busy) return false;
             return true;

@@ -4,9 +4,12 @@ p50/p99 latency (arrival to results ready) per offered rate.
 
   python -m ceos_amd.serve_bench --workload serve_10m --rates 20000,50000,100000
   python -m ceos_amd.serve_bench --workload stream_5m --rates 50000
+  python -m ceos_amd.serve_bench --workload gist_like --rates 50000,200000,400000
 
 serve_10m: 10,000,000 x 256 points (1,000-centre mixture), D=1024, R=4, m=2000;
 requests with k in {10,100}, n_probed_vectors 40, n_cand U[500,5000].
+gist_like: 1,000,000 x 960 points, D=1024, R=8, m=500; requests with the uneven batch's
+per-query k in {1,10,100}, n_probed_vectors U[10,80] and n_cand U[200,5000].
 stream_5m: 5,000,000 x 384 points, D=512, R=8, m=1000 in a streamCEOs index; requests
 with k=10, n_probed_vectors 40, n_cand 1000, while every 100 ms a batch of 10,000 new
 points is inserted and 0.5% of the live ids are deleted at random.
@@ -31,7 +34,7 @@ def log(msg):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--workload", default="serve_10m", choices=["serve_10m", "stream_5m"])
+    ap.add_argument("--workload", default="serve_10m", choices=["serve_10m", "stream_5m", "gist_like"])
     ap.add_argument("--rates", default="20000,50000,100000")
     ap.add_argument("--seconds", type=float, default=2.0, help="arrival window per rate")
     ap.add_argument("--n", type=int, default=0, help="override number of points (testing only)")
@@ -41,7 +44,7 @@ def main():
     args = ap.parse_args()
 
     from ceos_amd import CEOs
-    from ceos_amd.workloads import WORKLOADS, make_points, make_queries
+    from ceos_amd.workloads import WORKLOADS, make_points, make_queries, make_query_params
 
     w = WORKLOADS[args.workload]
     n = args.n or w.n
@@ -60,7 +63,13 @@ def main():
         index.build_coCEOs_Est(X)
     log(f"index built in {time.time() - t:.1f}s")
     index.n_probed_vectors, index.n_probed_points, index.n_cand = w.s, w.m, w.c
-    k_max, s_max, c_max = (10, w.s, w.c) if stream else (100, w.s, 5000)
+    uneven = bool(w.uneven)
+    if stream:
+        k_max, s_max, c_max = 10, w.s, w.c
+    elif uneven:
+        k_max, s_max, c_max = 100, 80, 5000
+    else:
+        k_max, s_max, c_max = 100, w.s, 5000
     srv = CEOs.Server(index, max_batch=args.max_batch, k_max=k_max, s_max=s_max, c_max=c_max, inflight=args.inflight)
     rng = np.random.default_rng(3)
     upd_rng = np.random.default_rng(4)
@@ -72,10 +81,13 @@ def main():
         if stream:
             k = np.full(nreq, 10, np.int32)
             c = np.full(nreq, w.c, np.int32)
+            s = np.full(nreq, w.s, np.int32)
+        elif uneven:
+            k, s, c = make_query_params(w, nreq, seed=2 + rate)
         else:
             k = rng.choice(np.array([10, 100], np.int32), nreq).astype(np.int32)
             c = rng.integers(500, 5001, nreq).astype(np.int32)
-        s = np.full(nreq, w.s, np.int32)
+            s = np.full(nreq, w.s, np.int32)
         arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
         srv.reset_stats()
         stop = threading.Event()

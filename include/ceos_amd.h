@@ -205,8 +205,8 @@ int ceos_stream_server_create(ceos_stream* h, const ceos_server_config* cfg, ceo
 /* Waits for every queued request, then stops the worker. */
 void ceos_server_destroy(ceos_server* s);
 /* Queues n requests (rows of Q, copied). arrival_s (optional) gives each request's
- * arrival time in seconds after this call (open-loop load); NULL = now. Tickets are
- * first_ticket .. first_ticket + n - 1. */
+ * arrival time in seconds after the submission is queued, i.e. after its rows have been
+ * copied (open-loop load); NULL = at once. Tickets are first_ticket .. first_ticket + n - 1. */
 int ceos_server_submit(ceos_server* s, const float* Q, int n, int d, const int32_t* k, const int32_t* n_probed_vectors,
                        const int32_t* n_cand, const double* arrival_s, int64_t* first_ticket);
 /* Results of one request: k entries into ids/ips (descending, zero padded), its k and
