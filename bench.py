@@ -8,6 +8,9 @@ and per-query (k, n_probed_vectors, n_cand) already resident in HBM. Reported be
 it, from the same run:
   * plain per-query GPU path: the same index searched one query per call,
     synchronising after each (the drop-in search used query by query);
+  * served: the continuous-batching server over the same index for about a second of
+    open-loop Poisson arrivals (60% of the batched rate by default), achieved QPS and
+    p50/p99 latency from arrival to results in host memory;
   * build throughput (Mpoints/s) of the device build from HBM-resident points;
   * recall@k against exact MIPS (FP32 matmul on the device);
   * roofline of the fused search kernel: algorithmic HBM bytes per launch over its
@@ -67,6 +70,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="repeat the CPU sample for this long")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-qps", type=float, default=-1.0,
+                    help="offered rate of the served run (default: 60%% of the batched rate; 0 = skip)")
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (counter calibration under a profiler)")
@@ -241,6 +247,27 @@ def main():
             f"p99 {np.percentile(lat, 99):.3f} ms")
         capi.check(L.ceos_set_query_param(h, w.s, w.m, w.c))
 
+    # ---- served: the continuous-batching server (two batch slots of up to 4096 queries)
+    # over the same index, open-loop Poisson arrivals of the workload's queries with their
+    # per-query parameters; latency from arrival to results in host memory
+    served = None
+    rate = args.serve_qps if args.serve_qps >= 0 else round(0.6 * qps, -3)
+    if rate > 0:
+        nreq = max(1, int(rate * args.serve_seconds))
+        idx = np.arange(nreq) % nq
+        srv = CEOs.Server(index, max_batch=4096, k_max=k_max, s_max=s_max, c_max=c_max, inflight=2)
+        arrivals = np.cumsum(np.random.default_rng(3).exponential(1.0 / rate, nreq))
+        srv.submit(Q[idx], ks[idx], ss[idx], cs[idx], arrivals)
+        srv.drain()
+        st = srv.stats()
+        srv.close()
+        served = {"offered_qps": rate, "achieved_qps": round(st["qps"], 1), "requests": nreq,
+                  "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
+                  "mean_batch": round(st["mean_batch"], 1), "inflight_slots": st["inflight_slots"],
+                  "vs_plain": round(st["qps"] / plain_qps, 1) if plain_qps else None}
+        log(f"served: {st['qps']:,.0f} QPS offered {rate:,.0f}, p50 {st['p50_ms']:.3f} ms p99 {st['p99_ms']:.3f} ms, "
+            f"mean batch {st['mean_batch']:.1f}")
+
     # ---- CPU baseline (oracle restatement of the reference search), same index
     cpu = None
     c_eff = cs.astype(np.float64)
@@ -333,6 +360,7 @@ def main():
         "recall_at_k": round(rec_k, 4),
         "plain_qps": round(plain_qps, 1) if plain_qps else None,
         "batched_vs_plain": round(qps / plain_qps, 1) if plain_qps else None,
+        "served": served,
         "plain_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                              "p99": round(float(np.percentile(lat, 99)), 4)},
         "build_mpts_s": round(n / build_s / 1e6, 2),
