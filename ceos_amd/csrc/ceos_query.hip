@@ -61,6 +61,12 @@ __host__ __device__ inline int tile_words(int L) {
     return L >= 8 ? (1 << L) + 64 : (1 << L);
 }
 
+// CEOs-Est candidate bound: estimates of kEstBlocks blocks of 256 points spread over the
+// data give a lower bound of the c-th largest estimate
+constexpr int kEstBlocks = 16;
+constexpr int kEstSample = kEstBlocks * kQNT;
+constexpr int kEstFlight = 8;  // columns loaded together per point group
+
 struct QLayout {
     int q_off, probe_off, pkey_off, misc_off, u_off;
     int tile_off, proj_off, tab_off, rk_off, tk_off, total;
@@ -82,8 +88,10 @@ __host__ __device__ inline QLayout q_layout(int d, int L, int DR, int tlog2, int
     o.rk_off = off;
     o.tk_off = off + align16(c_max * 8);
     const int endD = o.tk_off + next_pow2(k_max) * 8;
+    const int endS = off + kEstSample * 4;  // CEOs-Est sample keys (after the probe selection)
     o.total = endA > endB ? endA : endB;
     if (endD > o.total) o.total = endD;
+    if (endS > o.total) o.total = endS;
     return o;
 }
 
@@ -271,17 +279,110 @@ __global__ __launch_bounds__(kQNT) void k_est_search(IndexDims dm, const uint32_
         const float* qrow = qb.Q + (int64_t)q * dm.d;
         project_query<L>(dm, qrow, signs, tile, proj, qv);
         select_probes(DR, sq, proj, pkey, probes, sh);
-        for (int64_t i = tid; i < n; i += kQNT) {
+        // Estimates in probe order (the reference's vecEst += / -= per probe,
+        // src/CEOs.cpp:208-219), kEstFlight column loads in flight per point (group).
+        auto est1 = [&](int64_t i) -> float {
             float e = 0.0f;
-            for (int j = 0; j < sq; ++j) {
-                const int lj = probes[j];
-                const float v = Pm[(int64_t)(lj >> 1) * n + i];
-                e = (lj & 1) ? e - v : e + v;
+            for (int j0 = 0; j0 < sq; j0 += kEstFlight) {
+                float v[kEstFlight];
+#pragma unroll
+                for (int u = 0; u < kEstFlight; ++u)
+                    v[u] = j0 + u < sq ? Pm[(int64_t)(probes[j0 + u] >> 1) * n + i] : 0.0f;
+#pragma unroll
+                for (int u = 0; u < kEstFlight; ++u)
+                    if (j0 + u < sq) e = (probes[j0 + u] & 1) ? e - v[u] : e + v[u];
             }
-            est[i] = ckey(e, (uint32_t)i);
+            return e;
+        };
+        auto est4 = [&](int64_t i) -> float4 {  // points i..i+3; n % 4 == 0, i % 4 == 0
+            float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            for (int j0 = 0; j0 < sq; j0 += kEstFlight) {
+                float4 v[kEstFlight];
+#pragma unroll
+                for (int u = 0; u < kEstFlight; ++u)
+                    v[u] = j0 + u < sq ? *reinterpret_cast<const float4*>(Pm + (int64_t)(probes[j0 + u] >> 1) * n + i)
+                                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                for (int u = 0; u < kEstFlight; ++u) {
+                    if (j0 + u >= sq) continue;
+                    if (probes[j0 + u] & 1) {
+                        e.x -= v[u].x; e.y -= v[u].y; e.z -= v[u].z; e.w -= v[u].w;
+                    } else {
+                        e.x += v[u].x; e.y += v[u].y; e.z += v[u].z; e.w += v[u].w;
+                    }
+                }
+            }
+            return e;
+        };
+        const bool vec4 = (n & 3) == 0;
+        // lower bound of the cq-th largest estimate from the sample: the sample value of
+        // about twice the expected rank, plus 8 (none when the sample is not small against
+        // n, or cq is a large share of n)
+        uint32_t lkey = 0;
+        bool bounded = false;
+        const int64_t rs = 2 * (((int64_t)cq * kEstSample + n - 1) / n) + 8;
+        if (n >= 8 * (int64_t)kEstSample && 2 * rs <= kEstSample) {
+            uint32_t* skey = reinterpret_cast<uint32_t*>(lds + lo.u_off);
+            for (int b = 0; b < kEstBlocks; ++b) {
+                const int64_t start = ((int64_t)b * (n - kQNT) / (kEstBlocks - 1)) & ~(int64_t)3;
+                skey[b * kQNT + tid] = fkey(est1(start + tid));
+            }
+            __syncthreads();
+            uint64_t prefix;
+            int shift;
+            radix_select<kQNT>(kEstSample, (int)rs, [&](int i) -> uint64_t { return (uint64_t)skey[i] << 32; }, sh,
+                               prefix, shift, 32);
+            lkey = (uint32_t)(prefix >> 32);
+            bounded = true;
+        }
+        int nk = 0;
+        if (bounded) {  // keys of the points at or above the bound, appended
+            if (tid == 0) misc[1] = 0;
+            __syncthreads();
+            for (int64_t i0 = 0; i0 < n; i0 += 4 * kQNT) {  // uniform trip count (wave-level append)
+                const int64_t i = i0 + 4 * tid;
+                float e[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                int cnt = 0;
+                if (i < n) {
+                    if (vec4) {
+                        const float4 e4 = est4(i);
+                        e[0] = e4.x; e[1] = e4.y; e[2] = e4.z; e[3] = e4.w;
+                    } else {
+                        for (int u = 0; u < 4; ++u)
+                            if (i + u < n) e[u] = est1(i + u);
+                    }
+                }
+                uint32_t tm = 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (i + u < n && fkey(e[u]) >= lkey) tm |= 1u << u;
+                cnt = __popc(tm);
+                uint32_t pos = wave_append_n<3>((uint32_t)cnt, reinterpret_cast<uint32_t*>(&misc[1]));
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if ((tm >> u) & 1u) est[pos++] = ckey(e[u], (uint32_t)(i + u));
+            }
+            __syncthreads();
+            nk = misc[1];
+            __syncthreads();
+        }
+        if (!bounded || nk < cq) {  // every point's key, in place (rare when bounded)
+            for (int64_t i = 4 * (int64_t)tid; i < n; i += 4 * kQNT) {
+                if (vec4) {
+                    const float4 e4 = est4(i);
+                    est[i] = ckey(e4.x, (uint32_t)i);
+                    est[i + 1] = ckey(e4.y, (uint32_t)(i + 1));
+                    est[i + 2] = ckey(e4.z, (uint32_t)(i + 2));
+                    est[i + 3] = ckey(e4.w, (uint32_t)(i + 3));
+                } else {
+                    for (int u = 0; u < 4; ++u)
+                        if (i + u < n) est[i + u] = ckey(est1(i + u), (uint32_t)(i + u));
+                }
+            }
+            nk = (int)n;
         }
         __syncthreads();
-        const int ce = select_candidates(est, (int)n, cq, sel, sh);
+        const int ce = select_candidates(est, nk, cq, sel, sh);
         rerank(qv, dm.d, rows, sel, ce, rk);
         const int64_t off = qb.out_off ? qb.out_off[q] : (int64_t)q * qb.k0;
         write_topk(rk, ce, kq, tk, sh, qb.out_id + off, qb.out_ip + off);

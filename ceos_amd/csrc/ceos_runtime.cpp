@@ -286,7 +286,8 @@ int ceos_search_est(ceos_index* h, const float* Q, int nq, int d, int k, int32_t
         if (lds > kMaxLdsBytes) throw ArgError("n_cand too large for the device path");
         const int64_t stride = c.est_n + cc;
         int grid = device_cus() * 2;
-        grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t(2) << 30) / (stride * 8)));
+        // per-workgroup scratch holds n keys (the unbounded case); at most 4 GiB in all
+        grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t(4) << 30) / (stride * 8)));
         c.scr.cand.ensure((size_t)grid * stride);
         if (!c.scr.work.p) c.scr.work.alloc(1);
         DBuf<float> dQ((size_t)nq * d);

@@ -166,3 +166,27 @@ def test_search_candidate_select_paths(CEOs, oracle, n_cand):
     gi, gp = g.search_coCEOs_Est(Q, 10)
     oi, op = o.search_coCEOs_Est(Q, 10)
     _compare_search(gi, gp, oi, op, tol_frac=0.0)
+
+
+@pytest.mark.parametrize("n_points", [200_000, 200_001])
+def test_ceos_est_sampled_bound(CEOs, oracle, n_points):
+    """CEOs-Est over enough points that the device bounds the n_cand-th estimate from a
+    sample and keeps only the points at or above it; an odd point count takes the
+    scalar column path. Ids and inner products must equal the oracle's dense search."""
+    rng = np.random.default_rng(33)
+    d = 24
+    X = (rng.standard_normal((n_points, d)) * rng.lognormal(0, 0.5, (n_points, 1))).astype(np.float32)
+    Q = rng.standard_normal((100, d)).astype(np.float32)
+    Q /= np.linalg.norm(Q, axis=1, keepdims=True)
+    g = CEOs.CEOs(n_points, d)
+    g.setIndexParam(128, 2, 10, -1, 42)
+    g.build(X)
+    o = oracle.OracleCEOs(n_points, d)
+    o.setIndexParam(128, 2, 10, 8, 42)
+    o.build(X)
+    for obj in (g, o):
+        obj.n_probed_vectors = 12
+        obj.n_cand = 300
+    gi, gp = g.search(Q, 10)
+    oi, op = o.search(Q, 10)
+    _compare_search(gi, gp, oi, op, tol_frac=0.0)
