@@ -55,6 +55,79 @@ def exact_topk(torch, dX, dQ, k, chunk=256):
     return torch.cat(out).numpy()
 
 
+def run_mode(args, w, X, Q, dX, dQ, n, d, nq, torch, CEOs):
+    """CEOs-Est (`build`/`search`) or CEOs-Hash (`build_CEOs_Hash`/`search_CEOs_Hash`) on
+    the workload's points and queries with its n_probed_vectors / n_probed_points / n_cand
+    for every query. A step is one search call over all queries (host entry point: the
+    queries and results cross PCIe, about 1 MB each way, inside the timed region)."""
+    from oracle import oracle as O
+
+    mode = args.mode
+    k = 10
+    s, mp, c = w.s, (w.m if mode == "est" else args.hash_points), w.c
+    g = CEOs.CEOs(n, d)
+    g.setIndexParam(w.D, w.R, w.m, -1, w.seed)
+    build = g.build if mode == "est" else g.build_CEOs_Hash
+    build(X)
+    t0 = time.perf_counter()
+    build(X)  # the second build (host points, copied in as the reference does)
+    build_s = time.perf_counter() - t0
+    g.n_probed_vectors, g.n_probed_points, g.n_cand = s, mp, c
+    search = g.search if mode == "est" else g.search_CEOs_Hash
+    for _ in range(args.warmup):
+        ids, _ = search(Q, k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ids, _ = search(Q, k)
+    step_s = (time.perf_counter() - t0) / args.steps
+    qps = nq / step_s
+    log(f"{mode}: build {build_s * 1e3:.1f} ms, search {step_s * 1e3:.2f} ms/step, {qps:,.0f} QPS")
+    exact = exact_topk(torch, dX, dQ, k)
+    rec = float(np.mean([len(np.intersect1d(exact[q], ids[q])) / k for q in range(nq)]))
+    # CPU baseline: the oracle's search on a sample of the queries, same parameters
+    o = O.OracleCEOs(n, d)
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    o.setIndexParam(w.D, w.R, w.m, threads, w.seed)
+    (o.build if mode == "est" else o.build_CEOs_Hash)(X)
+    o.n_probed_vectors, o.n_probed_points, o.n_cand = s, mp, c
+    S = min(nq, 64 if mode == "est" else 1000)
+    if mode == "est":
+        oi, _ = o.search(Q[:S], k)
+        distinct = None
+    else:
+        oi, _, distinct = o.search_CEOs_Hash(Q[:S], k, stats=True)
+    reps, t0 = 1, time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds:
+        (o.search if mode == "est" else o.search_CEOs_Hash)(Q[:S], k)
+        reps += 1
+    cpu_qps = S * (reps - 1) / max(1e-9, time.perf_counter() - t0) if reps > 1 else None
+    match = float(np.mean([np.array_equal(oi[q], ids[q]) for q in range(S)]))
+    # algorithmic bytes per query: the query, the probed columns (CEOs-Est: s columns of n
+    # estimates) or lists (CEOs-Hash: s x n_probed_points ids), the reranked rows, the output
+    if mode == "est":
+        per_q = 4.0 * d + 4.0 * s * n + 4.0 * d * min(c, n) + 8 * k
+    else:
+        per_q = 4.0 * d + 8.0 * s * mp + 4.0 * d * float(np.mean(distinct)) + 8 * k
+    achieved = per_q * nq / step_s / 1e9
+    out = {
+        "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (numpy PCG64 seeds 0/1; index sign seed 42)",
+        "config": {"workload": w.name, "mode": "CEOs-Est" if mode == "est" else "CEOs-Hash", "n": n, "d": d,
+                   "queries": nq, "n_proj": w.D, "n_repeats": w.R, "top_m": w.m, "n_probed_vectors": s,
+                   "n_probed_points": mp, "n_cand": c, "k": k},
+        "recall_at_10": round(rec, 4), "build_ms": round(build_s * 1e3, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_est_search" if mode == "est" else "k_search_fused (hash mode)",
+                     "bytes_per_query": round(per_q), "timed": "host entry point (PCIe copies of Q and results included)"},
+        "cpu_baseline": {"value": round(cpu_qps, 1) if cpu_qps else None, "unit": "queries/s", "cores": threads,
+                         "kind": "port", "sample": f"{S} queries, {reps - 1} passes", "topk_match_vs_gpu": match},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +147,9 @@ def main():
                     help="offered rate of the served run (default: 60%% of the batched rate; 0 = skip)")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--mode", default="coceos", choices=["coceos", "est", "hash"],
+                    help="coceos (default): batched coCEOs; est: CEOs-Est build/search; hash: CEOs-Hash")
+    ap.add_argument("--hash-points", type=int, default=100, help="n_probed_points of the CEOs-Hash mode")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (counter calibration under a profiler)")
     args = ap.parse_args()
@@ -105,6 +181,9 @@ def main():
     log(f"generated data in {time.time() - t:.1f}s")
     dX = torch.from_numpy(X).to(dev)
     dQ = torch.from_numpy(Q).to(dev)
+    if args.mode != "coceos":
+        run_mode(args, w, X, Q, dX, dQ, n, d, nq, torch, CEOs)
+        return
     # a real (non-null) stream: the library launches on it and the events below bracket them
     ts = torch.cuda.Stream()
     torch.cuda.set_stream(ts)
