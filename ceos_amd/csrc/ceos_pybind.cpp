@@ -564,6 +564,12 @@ class PyServer {
         o["qps"] = st.qps;
         o["inflight_slots"] = st.inflight_slots;
         o["device_bytes"] = st.device_bytes;
+        o["inserts"] = st.inserts;
+        o["deletes"] = st.deletes;
+        o["insert_mean_ms"] = st.insert_mean_ms;
+        o["insert_max_ms"] = st.insert_max_ms;
+        o["delete_mean_ms"] = st.delete_mean_ms;
+        o["delete_max_ms"] = st.delete_max_ms;
         return o;
     }
     void reset_stats() { check(ceos_server_reset_stats(live())); }

@@ -116,6 +116,8 @@ struct ceos_server {
     // statistics
     int64_t submitted = 0, completed = 0, batches = 0, batched_queries = 0;
     std::vector<double> lat_ms;
+    int64_t n_upd[2] = {0, 0};                           // inserts, deletes applied
+    double upd_ms_sum[2] = {0.0, 0.0}, upd_ms_max[2] = {0.0, 0.0};
     Clock::time_point first_arrival{}, last_done{};
     bool any_arrival = false;
 
@@ -251,13 +253,19 @@ struct ceos_server {
 
     void apply_update(Update& u, std::unique_lock<std::mutex>& lk) {
         lk.unlock();
+        const auto t0 = Clock::now();
         int rc;
         if (u.is_delete)
             rc = ceos_stream_delete_ids(stream, u.ids.data(), (int64_t)u.ids.size());
         else
             rc = ceos_stream_update(stream, u.X.data(), u.n_new, core->d, u.n_del);
         const std::string err = rc == CEOS_OK ? std::string() : g_err;
+        const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
         lk.lock();
+        const int kind = u.is_delete ? 1 : 0;
+        ++n_upd[kind];
+        upd_ms_sum[kind] += ms;
+        upd_ms_max[kind] = std::max(upd_ms_max[kind], ms);
         if (rc != CEOS_OK) update_error = "stream update failed: " + err;  // searches go on
         cv_done.notify_all();
     }
@@ -586,6 +594,12 @@ int ceos_server_get_stats(ceos_server* s, ceos_server_stats* out) {
         }
         st.inflight_slots = (int)s->slots.size();
         st.device_bytes = s->device_bytes;
+        st.inserts = s->n_upd[0];
+        st.deletes = s->n_upd[1];
+        st.insert_mean_ms = s->n_upd[0] ? s->upd_ms_sum[0] / (double)s->n_upd[0] : 0.0;
+        st.insert_max_ms = s->upd_ms_max[0];
+        st.delete_mean_ms = s->n_upd[1] ? s->upd_ms_sum[1] / (double)s->n_upd[1] : 0.0;
+        st.delete_max_ms = s->upd_ms_max[1];
         *out = st;
     });
 }
@@ -597,6 +611,8 @@ int ceos_server_reset_stats(ceos_server* s) {
         s->submitted = s->completed = s->batches = s->batched_queries = 0;
         s->lat_ms.clear();
         s->any_arrival = false;
+        s->n_upd[0] = s->n_upd[1] = 0;
+        s->upd_ms_sum[0] = s->upd_ms_sum[1] = s->upd_ms_max[0] = s->upd_ms_max[1] = 0.0;
     });
 }
 

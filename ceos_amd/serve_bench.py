@@ -136,7 +136,11 @@ def main():
                "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
                "batches": st["batches"], "mean_batch": round(st["mean_batch"], 1),
                "inflight_slots": st["inflight_slots"], "device_bytes_slots": st["device_bytes"],
-               "alone_match": match, "updates": n_updates[0] if stream else None}
+               "alone_match": match, "updates": n_updates[0] if stream else None,
+               "insert_10k_mean_ms": round(st["insert_mean_ms"], 3) if stream else None,
+               "insert_10k_max_ms": round(st["insert_max_ms"], 3) if stream else None,
+               "delete_mean_ms": round(st["delete_mean_ms"], 3) if stream else None,
+               "delete_max_ms": round(st["delete_max_ms"], 3) if stream else None}
         log(json.dumps(out))
         print(json.dumps(out), flush=True)
     srv.close()

@@ -198,6 +198,8 @@ typedef struct {
     double qps;          /* completed / elapsed_s */
     int inflight_slots;
     int64_t device_bytes;
+    int64_t inserts, deletes;  /* streamCEOs updates applied */
+    double insert_mean_ms, insert_max_ms, delete_mean_ms, delete_max_ms;  /* time to apply one */
 } ceos_server_stats;
 
 int ceos_server_create(ceos_index* h, const ceos_server_config* cfg, ceos_server** out);
