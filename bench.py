@@ -142,6 +142,7 @@ def main():
     ap.add_argument("--cpu-queries", type=int, default=0, help="CPU baseline sample (default: all queries)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="repeat the CPU sample for this long")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-build-points", type=int, default=50_000, help="CPU build baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-qps", type=float, default=-1.0,
                     help="offered rate of the served run (default: 60%% of the batched rate; 0 = skip)")
@@ -214,7 +215,7 @@ def main():
     t0 = time.perf_counter()
     ids, _ = plan.search(Q, ks, ss, cs)  # rows of k_max: k[q] results then zeros
     log(f"plan first run (with autotune) {time.perf_counter() - t0:.2f}s; "
-        f"(wg/CU, tile log2, tuned ms) = {plan.config()}")
+        f"(wg/CU, tile log2, tuned ms) = {plan.config()}, projection waves/WG {plan.proj_waves()}")
     dk = torch.from_numpy(ks).to(dev)
     ds = torch.from_numpy(ss).to(dev)
     dc = torch.from_numpy(cs).to(dev)
@@ -375,10 +376,22 @@ def main():
         c_eff[:S] = np.minimum(cs[:S], distinct)
         log(f"cpu oracle: {cpu_qps:,.0f} QPS on {S} queries x {reps}, {threads} threads; top-k match {match:.4f}; "
             f"mean distinct candidates {distinct.mean():.0f}")
+        # the oracle's build (the reference's: all projections, then a heap per list) on a
+        # bounded sample of the same points and parameters
+        nb = min(n, max(w.m, args.cpu_build_points))
+        ob = O.OracleCEOs(nb, d)
+        ob.setIndexParam(w.D, w.R, w.m, threads, w.seed)
+        t0 = time.perf_counter()
+        ob.build_coCEOs_Est(X[:nb])
+        cpu_build_s = time.perf_counter() - t0
+        del ob
+        cpu_build = {"mpts_s": round(nb / cpu_build_s / 1e6, 4), "sample": f"first {nb} points", "cores": threads,
+                     "seconds": round(cpu_build_s, 2)}
+        log(f"cpu oracle build: {nb / cpu_build_s / 1e6:.4f} Mpts/s on {nb} points, {threads} threads")
         cpu = {"value": round(cpu_qps, 1), "unit": "queries/s", "cores": threads, "kind": "port",
                "sample": f"{S} of the {nq} queries, {reps} passes, on the same index (buckets from the device "
                          f"build, verified identical to the oracle's build in tests); {cpu_s:.1f} s",
-               "topk_match_vs_gpu": match}
+               "topk_match_vs_gpu": match, "build": cpu_build}
 
     # ---- roofline of the fused search kernel. Algorithmic bytes per query: the query row,
     # the probed bucket entries (8 B each), the candidates' rows and the top-k output;
@@ -413,6 +426,9 @@ def main():
                       "unit": "T add/s", "frac": round(adds_pt * n / build_s / 1e12 / VALU_ADD_PEAK_T, 4)},
              "hbm": {"achieved": round(build_bytes / build_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(build_bytes / build_s / 1e9 / HBM_PEAK_GBS, 4), "bytes": round(build_bytes)}}
+    if cpu:
+        build["cpu_oracle_mpts_s"] = cpu["build"]["mpts_s"]
+        build["vs_cpu_oracle"] = round(build["mpts_s"] / cpu["build"]["mpts_s"], 1)
 
     uneven = bool(w.uneven)
     out = {
@@ -434,7 +450,7 @@ def main():
                    "n_probed_vectors": f"per query U[{ss.min()},{s_max}]" if uneven else w.s,
                    "n_cand": f"per query U[{cs.min()},{c_max}]" if uneven else w.c,
                    "batch": "all queries in one hipGraph-captured plan (projection, probe select, fused search)",
-                   "launch": {"wg_per_cu": wg, "table_log2": tl}},
+                   "launch": {"wg_per_cu": wg, "table_log2": tl, "proj_waves_per_wg": plan.proj_waves()}},
         "recall_at_10": round(rec10, 4) if rec10 is not None else None,
         "recall_at_k": round(rec_k, 4),
         "plain_qps": round(plain_qps, 1) if plain_qps else None,

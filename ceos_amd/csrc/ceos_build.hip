@@ -49,20 +49,20 @@ constexpr int kBuildWaves = kBuildNT / kWave;
 
 __host__ __device__ inline int sign_words(int L) { return ((1 << L) + 31) / 32; }
 
-template <int L>
-__global__ __launch_bounds__(kBuildNT) void k_project_rows(IndexDims dm, const float* __restrict__ X,
-                                                           int64_t first, int64_t stride, int64_t nrows,
-                                                           const uint32_t* __restrict__ signs,
-                                                           float* __restrict__ out, int64_t ld_out,
-                                                           int col_major) {
+// NW waves per workgroup, each projecting one (row, repeat) at a time.
+template <int L, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_project_rows(IndexDims dm, const float* __restrict__ X,
+                                                             int64_t first, int64_t stride, int64_t nrows,
+                                                             const uint32_t* __restrict__ signs,
+                                                             float* __restrict__ out, int64_t ld_out,
+                                                             int col_major) {
     using P = Proj<L>;
     extern __shared__ float lds_all[];
     const int wave = threadIdx.x / kWave, ln = threadIdx.x % kWave;
     float* lds = lds_all + wave * P::kLdsWords;
     const int W = sign_words(L);
     const int64_t total = nrows * dm.R;
-    for (int64_t p = (int64_t)blockIdx.x * kBuildWaves + wave; p < total;
-         p += (int64_t)gridDim.x * kBuildWaves) {
+    for (int64_t p = (int64_t)blockIdx.x * NW + wave; p < total; p += (int64_t)gridDim.x * NW) {
         const int64_t k = p / dm.R;
         const int r = (int)(p - k * dm.R);
         const float* x = X + (first + k * stride) * dm.d;
@@ -82,18 +82,30 @@ __global__ __launch_bounds__(kBuildNT) void k_project_rows(IndexDims dm, const f
     }
 }
 
-hipError_t launch_project_rows(const IndexDims& dm, const float* X, int64_t first, int64_t stride,
-                               int64_t nrows, const uint32_t* signs, float* out, int64_t ld_out,
-                               bool col_major, hipStream_t st) {
-    if (nrows <= 0) return hipSuccess;
+template <int NW>
+static hipError_t launch_project_rows_nw(const IndexDims& dm, const float* X, int64_t first, int64_t stride,
+                                         int64_t nrows, const uint32_t* signs, float* out, int64_t ld_out,
+                                         bool col_major, hipStream_t st) {
     const int64_t waves = nrows * dm.R;
-    const int grid = (int)std::min<int64_t>((waves + kBuildWaves - 1) / kBuildWaves, 256 * 16);
+    const int grid = (int)std::min<int64_t>((waves + NW - 1) / NW, 256 * 64 / NW);
     CEOS_SWITCH_L(dm.L, {
-        const size_t lds = (size_t)kBuildWaves * Proj<LL>::kLdsWords * sizeof(float);
-        hipLaunchKernelGGL(k_project_rows<LL>, dim3(grid), dim3(kBuildNT), lds, st, dm, X, first, stride,
+        const size_t lds = (size_t)NW * Proj<LL>::kLdsWords * sizeof(float);
+        allow_lds(k_project_rows<LL, NW>, lds);
+        hipLaunchKernelGGL((k_project_rows<LL, NW>), dim3(grid), dim3(NW * kWave), lds, st, dm, X, first, stride,
                            nrows, signs, out, ld_out, (int)col_major);
     });
     return hipGetLastError();
+}
+
+hipError_t launch_project_rows(const IndexDims& dm, const float* X, int64_t first, int64_t stride,
+                               int64_t nrows, const uint32_t* signs, float* out, int64_t ld_out,
+                               bool col_major, hipStream_t st, int waves_per_wg) {
+    if (nrows <= 0) return hipSuccess;
+    switch (waves_per_wg) {
+        case 2: return launch_project_rows_nw<2>(dm, X, first, stride, nrows, signs, out, ld_out, col_major, st);
+        case 8: return launch_project_rows_nw<8>(dm, X, first, stride, nrows, signs, out, ld_out, col_major, st);
+        default: return launch_project_rows_nw<4>(dm, X, first, stride, nrows, signs, out, ld_out, col_major, st);
+    }
 }
 
 // ---------------------------------------------------------------------------------

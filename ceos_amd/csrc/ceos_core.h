@@ -663,7 +663,7 @@ struct ceos_plan {
     } snap{};
     bool tuned = false;
     double tuned_ms = 0.0;
-    int wg = 0, tl = 0;
+    int wg = 0, tl = 0, pw = 4;  // launch shape: workgroups per CU, tile log2, projection waves per workgroup
 
     ~ceos_plan() {
         drop();
@@ -679,10 +679,11 @@ struct ceos_plan {
         const Core& x = *core;
         return Snap{x.signs.p, x.bkt.p, x.bsize.p, x.X.p, x.cap, x.head, x.base, x.live, x.mp, x.m, x.D, x.R, x.L, x.d};
     }
-    void capture(int wg_, int tl_) {
+    void capture(int wg_, int tl_, int pw_) {
         drop();
         Core& x = *core;
         cfg = x.search_config_with(s_max, c_max, k_max, wg_, tl_);
+        cfg.proj_waves = pw_;
         const ceos::QueryScratch sc = scr.ensure(cfg, nq_max, x.DR());
         // k0 = k_max: rows at q * k_max, zero padded to k_max
         ceos::QueryBatch qb{Q.p, nq_max, k.p, s.p, c.p, k_max, 0, 0, x.mp, nullptr, ids.p, ips.p, nq_dev.p};
@@ -702,40 +703,47 @@ struct ceos_plan {
         snap = current();
         wg = wg_;
         tl = tl_;
+        pw = pw_;
     }
     void launch(int nq, hipStream_t st) {
         HIP_CHECK(ceos::launch_fill_i32(nq_dev.p, nq, 1, st));  // a kernel, outside the graph
         HIP_CHECK(hipGraphLaunch(exec, st));
     }
-    // Times each candidate launch shape once on this batch and keeps the fastest. Every
-    // candidate computes the same results, so the batch's outputs are valid afterwards.
+    // Times each candidate launch shape once on this batch and keeps the fastest. First
+    // the fused kernel's shape (workgroups per CU, score-tile size), then, with that, the
+    // query projection's waves per workgroup. Every candidate computes the same results,
+    // so the batch's outputs are valid afterwards.
     void tune(int nq, hipStream_t st) {
         Core& x = *core;
         const int t1 = x.search_config_with(s_max, c_max, k_max, 1, 0).table_log2;
         const int t2 = x.search_config_with(s_max, c_max, k_max, 2, 0).table_log2;
-        std::vector<std::pair<int, int>> cands = {{1, 0}, {2, 0}};
-        if (t1 - 1 >= 8) cands.push_back({1, t1 - 1});
-        if (t2 - 1 >= 8) cands.push_back({2, t2 - 1});
+        struct Shape { int wg, tl, pw; };
+        std::vector<Shape> cands = {{1, 0, 4}, {2, 0, 4}};
+        if (t1 - 1 >= 8) cands.push_back({1, t1 - 1, 4});
+        if (t2 - 1 >= 8) cands.push_back({2, t2 - 1, 4});
         hipEvent_t e0, e1;
         HIP_CHECK(hipEventCreate(&e0));
         HIP_CHECK(hipEventCreate(&e1));
         double best = 1e30;
-        std::pair<int, int> pick = cands[0];
-        try {
-            for (const auto& cd : cands) {
-                capture(cd.first, cd.second);
-                launch(nq, st);  // warm
-                HIP_CHECK(hipEventRecord(e0, st));
-                launch(nq, st);
-                HIP_CHECK(hipEventRecord(e1, st));
-                HIP_CHECK(hipEventSynchronize(e1));
-                float ms = 0.0f;
-                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) {
-                    best = ms;
-                    pick = cd;
-                }
+        Shape pick = cands[0];
+        auto time_shape = [&](const Shape& cd) {
+            capture(cd.wg, cd.tl, cd.pw);
+            launch(nq, st);  // warm
+            HIP_CHECK(hipEventRecord(e0, st));
+            launch(nq, st);
+            HIP_CHECK(hipEventRecord(e1, st));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                pick = cd;
             }
+        };
+        try {
+            for (const auto& cd : cands) time_shape(cd);
+            const Shape base = pick;
+            for (int pw_ : {2, 8}) time_shape({base.wg, base.tl, pw_});
         } catch (...) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
@@ -743,7 +751,7 @@ struct ceos_plan {
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        capture(pick.first, pick.second);
+        capture(pick.wg, pick.tl, pick.pw);
         tuned = true;
         tuned_ms = best;
     }
@@ -756,7 +764,7 @@ struct ceos_plan {
         if (nq < 0 || nq > nq_max) throw ArgError("batch of " + std::to_string(nq) + " exceeds the plan's nq_max");
         validate();
         if (nq == 0) return;
-        if (!exec || !(snap == current())) capture(wg, tl);
+        if (!exec || !(snap == current())) capture(wg, tl, pw);
         if ((flags & CEOS_PLAN_AUTOTUNE) && !tuned)
             tune(nq, st);
         else

@@ -33,9 +33,11 @@ struct RowRing {
 // ---------------- build ----------------
 // Projects rows row(k) = first + k*stride (k < nrows) for all R repeats.
 // col_major: out[col * ld_out + k]; else out[k * ld_out + col], col = r*D + j.
+// waves_per_wg: rows projected together per workgroup (2, 4 or 8; the query projection's
+// tuned launch shape).
 hipError_t launch_project_rows(const IndexDims& dm, const float* X, int64_t first, int64_t stride,
                                int64_t nrows, const uint32_t* signs, float* out, int64_t ld_out,
-                               bool col_major, hipStream_t st);
+                               bool col_major, hipStream_t st, int waves_per_wg = 4);
 
 // out[c * rows + r] = in[r * cols + c] (rows <= 65535 * 64).
 hipError_t launch_transpose(const float* in, int64_t rows, int64_t cols, float* out, hipStream_t st);
@@ -94,6 +96,7 @@ struct QueryConfig {
     int lds_bytes;     // dynamic LDS per workgroup
     int s_max, c_max, k_max;
     int64_t cand_stride;  // scratch entries per workgroup
+    int proj_waves = 4;   // query projection: waves per workgroup (2, 4 or 8)
     // optional kernel timing: recorded before the query projection and before and
     // after the fused search kernel (all null: no events)
     hipEvent_t ev[3];

@@ -94,6 +94,11 @@ class PyPlan {
         check(ceos_plan_config(p_, &wg, &tl, &ms));
         return py::make_tuple(wg, tl, ms);
     }
+    int proj_waves() const {
+        int pw = 0;
+        check(ceos_plan_proj_waves(p_, &pw));
+        return pw;
+    }
 
    private:
     ceos_plan* p_;
@@ -620,7 +625,8 @@ PYBIND11_MODULE(CEOs, m) {
              py::arg("n_cand"))
         .def("run", &PyPlan::run, py::arg("nq"), py::arg("stream") = 0)
         .def("buffers", &PyPlan::buffers)
-        .def("config", &PyPlan::config);
+        .def("config", &PyPlan::config)
+        .def("proj_waves", &PyPlan::proj_waves);
 
     py::class_<PyServer>(m, "Server")
         .def(py::init([](PyCEOs& ix, int max_batch, int k_max, int s_max, int c_max, int inflight, int64_t hbm_budget,

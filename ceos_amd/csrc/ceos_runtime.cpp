@@ -781,4 +781,11 @@ int ceos_plan_config(const ceos_plan* p, int* wg_per_cu, int* table_log2, double
     });
 }
 
+int ceos_plan_proj_waves(const ceos_plan* p, int* waves_per_wg) {
+    return guarded([&] {
+        if (!p || !waves_per_wg) throw ArgError("null argument");
+        *waves_per_wg = p->pw;
+    });
+}
+
 }  // extern "C"

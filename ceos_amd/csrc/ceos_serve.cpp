@@ -380,7 +380,7 @@ int make_server(Core& x, ceos_index* hi, ceos_stream* hs, const ceos_server_conf
                 // capture the slot's graph now and replay it once with an empty batch, so the
                 // first requests do not pay the capture and the graph upload
                 std::lock_guard<std::mutex> g(x.mu);
-                sl->plan->capture(0, 0);
+                sl->plan->capture(0, 0, 4);
                 sl->plan->launch(0, sl->st);
                 HIP_CHECK(hipStreamSynchronize(sl->st));
             }

@@ -171,6 +171,8 @@ int ceos_plan_search(ceos_plan* p, const float* Q, int nq, int d, const int32_t*
                      const int32_t* c, int32_t* ids, float* ips);
 /* Launch shape in use (after tuning) and the tuned kernel time in ms (0 if untuned). */
 int ceos_plan_config(const ceos_plan* p, int* wg_per_cu, int* table_log2, double* tuned_ms);
+/* The query projection's waves per workgroup in use (2, 4 or 8; tuned with the rest). */
+int ceos_plan_proj_waves(const ceos_plan* p, int* waves_per_wg);
 
 /* -------------------------------- serving --------------------------------
  * No reference counterpart: continuous batching over a resident index. Requests (one
