@@ -109,9 +109,9 @@ hipError_t launch_project_rows(const IndexDims& dm, const float* X, int64_t firs
 }
 
 // ---------------------------------------------------------------------------------
-// out[c * rows + r] = in[r * cols + c], through 64 x 64 LDS tiles (coalesced both ways).
+// out[c * ld_out + r] = in[r * cols + c], through 64 x 64 LDS tiles (coalesced both ways).
 __global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in, int64_t rows, int64_t cols,
-                                                   float* __restrict__ out) {
+                                                   float* __restrict__ out, int64_t ld_out) {
     __shared__ float tile[64][65];
     const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
@@ -122,15 +122,18 @@ __global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in,
     __syncthreads();
     for (int i = ty; i < 64; i += 4) {
         const int64_t c = c0 + i, r = r0 + tx;
-        if (r < rows && c < cols) out[c * rows + r] = tile[tx][i];
+        if (r < rows && c < cols) out[c * ld_out + r] = tile[tx][i];
     }
 }
 
-hipError_t launch_transpose(const float* in, int64_t rows, int64_t cols, float* out, hipStream_t st) {
+hipError_t launch_transpose(const float* in, int64_t rows, int64_t cols, float* out, hipStream_t st,
+                            int64_t ld_out) {
     if (rows <= 0 || cols <= 0) return hipSuccess;
-    if ((rows + 63) / 64 > 65535) return hipErrorInvalidValue;
+    if ((rows + 63) / 64 > 65535 || (cols + 63) / 64 > 0x7fffffff) return hipErrorInvalidValue;
+    if (ld_out <= 0) ld_out = rows;
+    if (ld_out < rows) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_transpose, dim3((unsigned)((cols + 63) / 64), (unsigned)((rows + 63) / 64)), dim3(256), 0,
-                       st, in, rows, cols, out);
+                       st, in, rows, cols, out, ld_out);
     return hipGetLastError();
 }
 

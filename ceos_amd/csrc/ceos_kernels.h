@@ -39,8 +39,9 @@ hipError_t launch_project_rows(const IndexDims& dm, const float* X, int64_t firs
                                int64_t nrows, const uint32_t* signs, float* out, int64_t ld_out,
                                bool col_major, hipStream_t st, int waves_per_wg = 4);
 
-// out[c * rows + r] = in[r * cols + c] (rows <= 65535 * 64).
-hipError_t launch_transpose(const float* in, int64_t rows, int64_t cols, float* out, hipStream_t st);
+// out[c * ld_out + r] = in[r * cols + c] (rows <= 65535 * 64; ld_out >= rows, 0 = rows).
+hipError_t launch_transpose(const float* in, int64_t rows, int64_t cols, float* out, hipStream_t st,
+                            int64_t ld_out = 0);
 
 // Per list (2*DR lists: list 2c = close, 2c+1 = far) threshold = rank-th largest of the
 // column's sample values (far: of -value). sample is column-major [DR][S].

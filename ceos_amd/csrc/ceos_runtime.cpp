@@ -264,7 +264,18 @@ int ceos_build_est(ceos_index* h, const float* X, int64_t n, int d) {
         c.make_signs();
         c.estP.alloc((size_t)c.DR() * n);
         c.est_n = n;
-        HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p, 0, 1, n, c.signs.p, c.estP.p, n, true, c.st()));
+        // column-major [DR][n]: chunks of rows projected row-major (coalesced stores) into a
+        // workspace of at most 1 GiB, then transposed into place through LDS tiles
+        const int64_t DR_ = c.DR();
+        const int64_t chunk = std::min<int64_t>(
+            {n, int64_t(65535) * 64, std::max<int64_t>(64, ((int64_t(1) << 28) / DR_) & ~int64_t(63))});
+        DBuf<float> tmp((size_t)chunk * DR_);
+        for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+            const int64_t rows = std::min(chunk, n - r0);
+            HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p + r0 * c.d, 0, 1, rows, c.signs.p, tmp.p, DR_, false,
+                                                c.st()));
+            HIP_CHECK(ceos::launch_transpose(tmp.p, rows, DR_, c.estP.p + r0, c.st(), n));
+        }
         HIP_CHECK(hipStreamSynchronize(c.st()));
         c.built = true;
     });
