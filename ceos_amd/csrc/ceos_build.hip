@@ -491,6 +491,22 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
         p[i] = v;
 }
 
+// Clears the bits of ring slots (slot0 + j) mod cap, j < count (bitmap of 32-bit words).
+__global__ void k_clear_slot_bits(uint32_t* __restrict__ bits, int64_t slot0, int64_t count, int64_t cap) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (int64_t)gridDim.x * blockDim.x) {
+        int64_t sl = slot0 + j;
+        if (sl >= cap) sl -= cap;
+        atomicAnd(&bits[sl >> 5], ~(1u << (sl & 31)));
+    }
+}
+
+hipError_t launch_clear_slot_bits(uint32_t* bits, int64_t slot0, int64_t count, int64_t cap, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    const int grid = (int)std::min<int64_t>((count + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_clear_slot_bits, dim3(grid), dim3(256), 0, st, bits, slot0, count, cap);
+    return hipGetLastError();
+}
+
 // Streaming-read calibration: every thread sums 16-byte loads over a grid-stride loop
 // (four in flight) and writes one float. n4 = number of float4 elements.
 __global__ __launch_bounds__(256) void k_stream_read(const float4* __restrict__ x, int64_t n4,

@@ -141,6 +141,9 @@ int est_search_lds_bytes(const IndexDims& dm, int s_max, int c_max, int k_max);
 
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 
+// Clears bit (slot0 + j) mod cap of a slot bitmap for j < count (count <= cap).
+hipError_t launch_clear_slot_bits(uint32_t* bits, int64_t slot0, int64_t count, int64_t cap, hipStream_t st);
+
 // Streaming read of n floats (n % 4 == 0, x 16-byte aligned) by grid x 256 threads, one
 // partial sum per thread into out (grid * 256 floats).
 hipError_t launch_stream_read(const float* x, int64_t n, float* out, int grid, hipStream_t st);

@@ -597,19 +597,21 @@ int ceos_stream_update(ceos_stream* h, const float* Xn, int64_t n_new, int d, in
             if (n_new > seg1)
                 HIP_CHECK(hipMemcpyAsync(c.X.p, Xn + seg1 * c.d, (size_t)(n_new - seg1) * c.d * 4,
                                          hipMemcpyHostToDevice, st));
-            std::vector<uint32_t> dh(c.dead.n);
-            HIP_CHECK(hipMemcpyAsync(dh.data(), c.dead.p, dh.size() * 4, hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipStreamSynchronize(st));
-            for (int64_t j = 0; j < n_new; ++j) {
-                int64_t sl = slot0 + j;
-                if (sl >= c.cap) sl -= c.cap;
-                dh[sl >> 5] &= ~(1u << (sl & 31));
-            }
-            HIP_CHECK(hipMemcpyAsync(c.dead.p, dh.data(), dh.size() * 4, hipMemcpyHostToDevice, st));
+            if (c.dead.p) HIP_CHECK(ceos::launch_clear_slot_bits(c.dead.p, slot0, n_new, c.cap, st));
         }
-        // project the new rows: column-major [DR][n_new]
+        // project the new rows: column-major [DR][n_new], through a row-major projection
+        // (coalesced stores) and an LDS-tiled transpose
         DBuf<float> np((size_t)c.DR() * std::max<int64_t>(n_new, 1));
-        if (n_new > 0) {
+        if (n_new > 0 && n_new <= int64_t(65535) * 64) {
+            DBuf<float> tmp((size_t)c.DR() * n_new);
+            HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p + slot0 * c.d, 0, 1, seg1, c.signs.p, tmp.p, c.DR(),
+                                                false, st));
+            if (n_new > seg1)
+                HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p, 0, 1, n_new - seg1, c.signs.p,
+                                                    tmp.p + seg1 * c.DR(), c.DR(), false, st));
+            HIP_CHECK(ceos::launch_transpose(tmp.p, n_new, c.DR(), np.p, st, n_new));
+            HIP_CHECK(hipStreamSynchronize(st));  // tmp is freed on return
+        } else if (n_new > 0) {
             HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p + slot0 * c.d, 0, 1, seg1, c.signs.p, np.p, n_new,
                                                 true, st));
             if (n_new > seg1)
