@@ -111,15 +111,37 @@ __global__ __launch_bounds__(kPNT) void k_probe_select(int DR, const float* __re
                                                        const int32_t* __restrict__ s_arr, int s0, int s_stride,
                                                        int32_t* __restrict__ probes) {
     __shared__ SelectShared sh;
-    extern __shared__ uint64_t pkey[];  // next_pow2(s_stride) keys, then (STAGE) the DR projections
-    float* pl = reinterpret_cast<float*>(pkey + next_pow2(s_stride));
+    extern __shared__ uint64_t pkey[];  // next_pow2(s_stride) keys (at least 2), then (STAGE) the DR projections
+    float* pl = reinterpret_cast<float*>(pkey + max(2, next_pow2(s_stride)));  // 16-byte aligned
     if (nq_dev) nq = min(nq, nq_dev[0]);
     for (int q = blockIdx.x; q < nq; q += gridDim.x) {
         const int sq = min(max(s_arr ? s_arr[q] : s0, 0), min(s_stride, DR));
         if (sq == 0) continue;  // uniform
         const float* pq = qproj + (int64_t)q * DR;
-        if (STAGE) {
-            for (int c = threadIdx.x; c < DR; c += kPNT) pl[c] = pq[c];
+        if (STAGE) {  // all of a thread's loads in flight together
+            constexpr int U = 8;
+            if ((DR & 3) == 0) {
+                const int n4 = DR >> 2;
+                for (int c0 = threadIdx.x; c0 < n4; c0 += U * kPNT) {
+                    float4 t[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        t[u] = c0 + u * kPNT < n4 ? reinterpret_cast<const float4*>(pq)[c0 + u * kPNT]
+                                                  : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (c0 + u * kPNT < n4) reinterpret_cast<float4*>(pl)[c0 + u * kPNT] = t[u];
+                }
+            } else {
+                for (int c0 = threadIdx.x; c0 < DR; c0 += U * kPNT) {
+                    float t[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) t[u] = c0 + u * kPNT < DR ? pq[c0 + u * kPNT] : 0.0f;
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (c0 + u * kPNT < DR) pl[c0 + u * kPNT] = t[u];
+                }
+            }
             __syncthreads();
         }
         const float* src = STAGE ? pl : pq;
@@ -859,7 +881,7 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
     if (err != hipSuccess) return err;
     {
         const int pgrid = std::min(qb.nq, 4096);
-        const size_t klds = (size_t)next_pow2(cfg.s_max) * 8;
+        const size_t klds = (size_t)std::max(2, next_pow2(cfg.s_max)) * 8;
         if (DR <= kProbeStageMax) {
             const size_t plds = klds + (size_t)DR * 4;
             allow_lds_s(k_probe_select<true>, plds);
