@@ -8,7 +8,8 @@
 //    (src/fht_avx.c:1012-1252), so the result is bit-identical to it.
 //  * radix_select: block-wide exact top-K over 64-bit keys (8-bit digits, early
 //    exit once the boundary digit bucket is taken whole).
-//  * bitonic_sort_desc: block-wide sort of a power-of-two LDS array.
+//  * bitonic_sort_desc: block-wide descending sort of an LDS array (rank counting up
+//    to the block size, a bitonic network above).
 //  * Keys: fkey() maps an FP32 value to a uint32 whose unsigned order is the
 //    float order; composite keys put the value in the high word and ~id in the
 //    low word, so descending key order is (value desc, id asc) and keys are unique.
@@ -404,7 +405,7 @@ __device__ __forceinline__ void compact_selected(int n, GetKey get, uint64_t pre
 
 // Descending bitonic sort of N (power of two) 64-bit keys in LDS.
 template <int NT>
-__device__ __forceinline__ void bitonic_sort_desc(uint64_t* s, int N) {
+__device__ __forceinline__ void bitonic_sort_network(uint64_t* s, int N) {
     const int tid = threadIdx.x;
     for (int size = 2; size <= N; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -418,6 +419,32 @@ __device__ __forceinline__ void bitonic_sort_desc(uint64_t* s, int N) {
             __syncthreads();
         }
     }
+}
+
+// Descending sort of N 64-bit keys in LDS. Up to NT keys, by rank: the key of thread i
+// moves to the count of keys greater than it plus equal keys before it (a permutation
+// for any keys, ties keep their order), two barriers instead of the network's
+// log2(N)(log2(N)+1)/2. Larger N (a power of two) takes the bitonic network.
+template <int NT>
+__device__ __forceinline__ void bitonic_sort_desc(uint64_t* s, int N) {
+    if (N > NT) {
+        bitonic_sort_network<NT>(s, N);
+        return;
+    }
+    const int tid = threadIdx.x;
+    uint64_t k = 0;
+    int r = 0;
+    if (tid < N) {
+        k = s[tid];
+#pragma unroll 4
+        for (int j = 0; j < N; ++j) {
+            const uint64_t x = s[j];
+            r += (x > k) | ((x == k) & (j < tid));
+        }
+    }
+    __syncthreads();
+    if (tid < N) s[r] = k;
+    __syncthreads();
 }
 
 __host__ __device__ __forceinline__ int next_pow2(int v) {
