@@ -148,6 +148,8 @@ def main():
                     help="offered rate of the served run (default: 60%% of the batched rate; 0 = skip)")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-rerank-bounds", action="store_true",
+                    help="keep no int8 row copies: every candidate gets the exact FP32 rerank")
     ap.add_argument("--mode", default="coceos", choices=["coceos", "est", "hash"],
                     help="coceos (default): batched coCEOs; est: CEOs-Est build/search; hash: CEOs-Hash")
     ap.add_argument("--hash-points", type=int, default=100, help="n_probed_points of the CEOs-Hash mode")
@@ -193,6 +195,7 @@ def main():
 
     index = CEOs.CEOs(n, d)
     index.setIndexParam(w.D, w.R, w.m, -1, w.seed)
+    index.rerank_bounds = not args.no_rerank_bounds
     h = index._handle
     torch.cuda.synchronize()
     # build: the first build also pays one-time allocations; time the second
@@ -395,9 +398,14 @@ def main():
 
     # ---- roofline of the fused search kernel. Algorithmic bytes per query: the query row,
     # the probed bucket entries (8 B each), the candidates' rows and the top-k output;
-    # one launch covers all nq queries.
+    # one launch covers all nq queries. With rerank bounds (k <= 256, 4 k <= candidates)
+    # a candidate costs its int8 row and 16 B of row parameters, plus the FP32 rows of
+    # the k best (the candidates past k that survive the bound are not counted).
     mp = w.m
-    bytes_launch = float(np.sum(4 * d + 8 * ss.astype(np.float64) * mp + 4 * d * c_eff + 8 * ks))
+    kf = ks.astype(np.float64)
+    pruned = bool(index.rerank_bounds) & (kf >= 1) & (kf <= 256) & (4 * kf <= c_eff)
+    rerank_bytes = np.where(pruned, (d + 16) * c_eff + 4 * d * kf, 4 * d * c_eff)
+    bytes_launch = float(np.sum(4 * d + 8 * ss.astype(np.float64) * mp + rerank_bytes + 8 * ks))
     achieved = bytes_launch / (fused_ms / 1e3) / 1e9
     # measured HBM bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel on
     # this workload, scaled by a 1 GiB copy (profiles/k_search_fused_<workload>_traffic.json)
@@ -416,10 +424,13 @@ def main():
 
     # ---- build roofline: the FWHT adds of every point (R repeats x 3 rounds x F log2 F) against
     # the non-packed FP32 add peak; HBM bytes = the points read once (4d per point) plus the
-    # buckets written (2 lists x D*R columns x top_m entries x 8 B)
+    # buckets written (2 lists x D*R columns x top_m entries x 8 B), plus with rerank bounds
+    # the points read again and their int8 copies and row parameters written (5d + 16)
     F = 1 << (w.D - 1).bit_length() if w.D >= d else 1 << (d - 1).bit_length()
     adds_pt = w.R * 3 * F * (F.bit_length() - 1)
     build_bytes = 4.0 * d * n + 8.0 * 2 * w.D * w.R * w.m
+    if index.rerank_bounds:
+        build_bytes += (5.0 * d + 16) * n
     build = {"mpts_s": round(n / build_s / 1e6, 2), "ms": round(build_s * 1e3, 2),
              "adds_per_point": adds_pt,
              "valu": {"achieved": round(adds_pt * n / build_s / 1e12, 3), "peak": VALU_ADD_PEAK_T,

@@ -43,6 +43,8 @@ def lib():
             "ceos_search_batched_device": (i32, [vp, P, i32, i32, P, P, P, P, i32, i32, i32, P, P, vp]),
             "ceos_set_search_config": (i32, [vp, i32, i32]),
             "ceos_get_search_config": (i32, [vp, P, P, P]),
+            "ceos_set_rerank_bounds": (i32, [vp, i32]),
+            "ceos_get_rerank_bounds": (i32, [vp, P]),
             "ceos_get_stream": (vp, [vp]),
             "ceos_plan_create": (i32, [vp, i32, i32, i32, i32, i32, C.POINTER(C.c_void_p)]),
             "ceos_stream_plan_create": (i32, [vp, i32, i32, i32, i32, i32, C.POINTER(C.c_void_p)]),
@@ -112,6 +114,14 @@ class DeviceIndex:
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         check(lib().ceos_get_search_config(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def set_rerank_bounds(self, on: bool):
+        check(lib().ceos_set_rerank_bounds(self.h, 1 if on else 0))
+
+    def rerank_bounds(self) -> bool:
+        v = C.c_int()
+        check(lib().ceos_get_rerank_bounds(self.h, C.byref(v)))
+        return bool(v.value)
 
     def stream(self) -> int:
         return lib().ceos_get_stream(self.h) or 0

@@ -14,6 +14,8 @@
 // below the threshold; the host verifies the counts and rescans lists that missed.
 #include <algorithm>
 
+#include <cfloat>
+
 #include "ceos_device.h"
 #include "ceos_kernels.h"
 
@@ -504,6 +506,115 @@ hipError_t launch_clear_slot_bits(uint32_t* bits, int64_t slot0, int64_t count, 
     if (count <= 0) return hipSuccess;
     const int grid = (int)std::min<int64_t>((count + 255) / 256, 4096);
     hipLaunchKernelGGL(k_clear_slot_bits, dim3(grid), dim3(256), 0, st, bits, slot0, count, cap);
+    return hipGetLastError();
+}
+
+// Rerank bounds: an int8 copy c of each row and (s, r1, r2, 0) per slot, one wave per
+// row. s = max_i |x_i| / 127 and c_i = round(x_i / s), clamped to [-127, 127];
+//   r1 = |x - s c|_2 + g |x|_2,  r2 = s |c|_2,  g = n u / (1 - n u), u = 2^-24, n = d + 64,
+// evaluated in FP64 and rounded up to FP32. For a query q, the FP32 inner product v the
+// rerank forms (a summation of depth below n) satisfies |v - <q,x>| <= g |q||x|, and
+// |<q,x> - s <q,c>| <= |q| |x - s c| (Cauchy-Schwarz), so v is within |q| r1 of s <q,c>;
+// r2 scales the error of an approximation of <q,c> (rounding, or a coded query q^:
+// |s <q - q^, c>| <= |q - q^| r2). A row with a non-finite entry gets s = 0, c = 0,
+// r1 = +inf, r2 = 0.
+// G = lanes per row (the power of two covering d/4 float4 units, at most 64); a lane
+// keeps its first four units in registers between the two passes over the row.
+__host__ __device__ inline int quantize_lanes(int d) {
+    int G = 1;
+    while (G < (d >> 2) && G < kWave) G <<= 1;
+    return G;
+}
+
+__global__ __launch_bounds__(256) void k_quantize_rows(const float* __restrict__ X, int64_t slot0, int64_t count,
+                                                       int64_t cap, int d, int8_t* __restrict__ Xq,
+                                                       float4* __restrict__ qsr) {
+    const int ln = threadIdx.x % kWave;
+    const int units = d >> 2;
+    const int G = quantize_lanes(d);
+    const int gl = ln % G;
+    const int64_t j = ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * (kWave / G) + ln / G;
+    const bool ok = j < count;  // lanes of a missing row still join the group reductions
+    int64_t sl = slot0 + (ok ? j : 0);
+    if (sl >= cap) sl -= cap;
+    const float4* x4 = reinterpret_cast<const float4*>(X + sl * d);
+    uint32_t* c4 = reinterpret_cast<uint32_t*>(Xq + sl * d);
+    constexpr int KEEP = 4;
+    float4 keep[KEEP];
+    float mx = 0.0f;
+    bool fin = true;
+    auto see = [&](const float4 v) {
+        const float a = fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w)));
+        fin &= a <= FLT_MAX;  // false for inf and NaN
+        mx = fmaxf(mx, a);
+    };
+#pragma unroll
+    for (int k = 0; k < KEEP; ++k) {
+        const int uu = gl + k * G;
+        keep[k] = (ok && uu < units) ? x4[uu] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        see(keep[k]);
+    }
+    if (ok)
+        for (int uu = gl + KEEP * G; uu < units; uu += G) see(x4[uu]);
+    int bad = fin ? 0 : 1;
+    for (int o = G >> 1; o > 0; o >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, o, kWave));
+        bad |= __shfl_xor(bad, o, kWave);
+    }
+    fin = bad == 0;
+    const float s = (fin && mx > 0.0f) ? mx / 127.0f : 0.0f;
+    const float inv = s > 0.0f ? 127.0f / mx : 0.0f;
+    double e2 = 0.0, x2 = 0.0;
+    int c2 = 0;
+    auto code = [&](float v) {
+        int ci = 0;
+        if (s > 0.0f) ci = min(127, max(-127, (int)rintf(v * inv)));
+        const double dv = (double)v - (double)s * (double)ci;
+        e2 = fma(dv, dv, e2);
+        x2 = fma((double)v, (double)v, x2);
+        c2 += ci * ci;
+        return (uint32_t)ci & 0xffu;
+    };
+    auto put = [&](int uu, const float4 v) {
+        const uint32_t w = code(v.x) | (code(v.y) << 8) | (code(v.z) << 16) | (code(v.w) << 24);
+        c4[uu] = w;
+    };
+    if (ok) {
+#pragma unroll
+        for (int k = 0; k < KEEP; ++k)
+            if (gl + k * G < units) put(gl + k * G, keep[k]);
+        for (int uu = gl + KEEP * G; uu < units; uu += G) put(uu, x4[uu]);
+    }
+    for (int o = G >> 1; o > 0; o >>= 1) {
+        e2 += __shfl_xor(e2, o, kWave);
+        x2 += __shfl_xor(x2, o, kWave);
+        c2 += __shfl_xor(c2, o, kWave);
+    }
+    if (ok && gl == 0) {
+        auto up = [](double v) {
+            float f = (float)v;
+            if ((double)f < v) f = nextafterf(f, INFINITY);
+            return f;
+        };
+        float r1 = INFINITY, r2 = 0.0f;
+        if (fin) {
+            const double n = (double)d + 64.0, u = 0x1p-24;
+            const double g = n * u / (1.0 - n * u);
+            r1 = up((sqrt(e2) + g * sqrt(x2)) * (1.0 + 0x1p-30));
+            r2 = up((double)s * sqrt((double)c2) * (1.0 + 0x1p-30));
+        }
+        qsr[sl] = make_float4(s, r1, r2, 0.0f);
+    }
+}
+
+hipError_t launch_quantize_rows(const float* X, int64_t slot0, int64_t count, int64_t cap, int d, int8_t* Xq,
+                                float4* qsr, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    if (count > cap || slot0 < 0 || slot0 >= cap || d <= 0 || (d & 3) != 0) return hipErrorInvalidValue;
+    const int64_t per_block = 4 * (kWave / quantize_lanes(d));
+    const int64_t blocks = (count + per_block - 1) / per_block;
+    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_quantize_rows, dim3((unsigned)blocks), dim3(256), 0, st, X, slot0, count, cap, d, Xq, qsr);
     return hipGetLastError();
 }
 

@@ -191,6 +191,12 @@ struct Core {
     int64_t cap = 0, head = 0;
     uint32_t base = 0, live = 0;
     DBuf<uint32_t> dead;  // stream random-delete bitmap over ring slots
+    // rerank bounds: int8 row copies [cap][d] and per-slot (s, r1, r2, 0), kept in step
+    // with X for coCEOs and stream indexes (see k_quantize_rows); off: not kept, every
+    // candidate gets the exact rerank
+    DBuf<int8_t> Xq;
+    DBuf<float4> Xqs;
+    bool rerank_bounds = true;
 
     // query scratch
     ScratchSet scr;
@@ -260,6 +266,7 @@ struct Core {
 
     void clear() {
         bkt.release(); bsize.release(); X.release(); signs.release(); dead.release();
+        Xq.release(); Xqs.release();
         scr.release(); estP.release();
         bw_rows.release(); bw_sample.release(); bw_thr.release(); bw_base.release(); bw_cap.release();
         bw_count.release(); bw_ent.release();
@@ -289,6 +296,8 @@ struct Core {
 
     // Copies n rows (host or device pointer) into a fresh ring starting at slot 0.
     void load_points(const float* src, int64_t n, int64_t capacity, bool src_device, hipStream_t s_) {
+        Xq.release();
+        Xqs.release();
         X.alloc((size_t)capacity * d);
         cap = capacity; head = 0; base = 0; live = (uint32_t)n;
         HIP_CHECK(hipMemcpyAsync(X.p, src, (size_t)n * d * 4,
@@ -413,7 +422,35 @@ struct Core {
 
     ceos::QueryScratch scratch(const ceos::QueryConfig& cfg, int nq) { return scr.ensure(cfg, nq, DR()); }
 
-    ceos::RowRing rows() const { return ceos::RowRing{X.p, cap, head, base, live}; }
+    ceos::RowRing rows() const { return ceos::RowRing{X.p, cap, head, base, live, Xq.p, Xqs.p}; }
+
+    // Brings the rerank bounds up to date after rows were written at ring slots
+    // [slot0, slot0 + count) mod cap. When the copies are missing or sized for another
+    // ring, they are (re)made for the `total` rows from slot `head`. Needs d % 4 == 0.
+    void quantize_rows(int64_t slot0, int64_t count, int64_t total, hipStream_t s_) {
+        if (!rerank_bounds || d % 4 != 0 || cap <= 0) {
+            Xq.release();
+            Xqs.release();
+            return;
+        }
+        if (Xq.n != (size_t)cap * d || Xqs.n != (size_t)cap) {
+            Xq.alloc((size_t)cap * d);
+            Xqs.alloc((size_t)cap);
+            slot0 = head;
+            count = total;
+        }
+        HIP_CHECK(ceos::launch_quantize_rows(X.p, slot0, count, cap, d, Xq.p, Xqs.p, s_));
+    }
+    void set_rerank_bounds(bool on) {
+        rerank_bounds = on;
+        if (!on) {
+            Xq.release();
+            Xqs.release();
+        } else if (built && bkt.p && X.p) {
+            quantize_rows(head, live, live, st());
+            HIP_CHECK(hipStreamSynchronize(st()));
+        }
+    }
 
     void check_status() {
         int32_t flag = 0;
@@ -552,6 +589,8 @@ struct Core {
         seed = (int)(seed_v & 0x7fffffffu) == (int)seed_v ? (int)seed_v : -1;
         params_set = true;
         estP.release();
+        Xq.release();
+        Xqs.release();
         X.alloc((size_t)std::max<int64_t>(n, 1) * d);
         cap = std::max<int64_t>(n, 1); head = 0; base = 0; live = (uint32_t)n;
         const int64_t chunk = std::max<int64_t>(1, (int64_t(64) << 20) / (4LL * d));
@@ -586,6 +625,8 @@ struct Core {
             HIP_CHECK(hipMemcpy(dead.p, flags.data(), flags.size() * 4, hipMemcpyHostToDevice));
         }
         make_signs_from(seed_v);
+        quantize_rows(0, n, n, st());
+        HIP_CHECK(hipStreamSynchronize(st()));
         built = true;
         return n;
     }
@@ -651,12 +692,12 @@ struct ceos_plan {
     hipStream_t cap = nullptr;
     // everything the captured launches hold by value
     struct Snap {
-        const void *signs, *bkt, *bsize, *X;
+        const void *signs, *bkt, *bsize, *X, *Xq;
         int64_t cap, head;
         uint32_t base, live;
         int mp, m, D, R, L, d;
         bool operator==(const Snap& o) const {
-            return signs == o.signs && bkt == o.bkt && bsize == o.bsize && X == o.X && cap == o.cap &&
+            return signs == o.signs && bkt == o.bkt && bsize == o.bsize && X == o.X && Xq == o.Xq && cap == o.cap &&
                    head == o.head && base == o.base && live == o.live && mp == o.mp && m == o.m && D == o.D &&
                    R == o.R && L == o.L && d == o.d;
         }
@@ -677,7 +718,8 @@ struct ceos_plan {
     }
     Snap current() const {
         const Core& x = *core;
-        return Snap{x.signs.p, x.bkt.p, x.bsize.p, x.X.p, x.cap, x.head, x.base, x.live, x.mp, x.m, x.D, x.R, x.L, x.d};
+        return Snap{x.signs.p, x.bkt.p, x.bsize.p, x.X.p, x.Xq.p, x.cap, x.head, x.base, x.live, x.mp, x.m, x.D, x.R,
+                    x.L, x.d};
     }
     void capture(int wg_, int tl_, int pw_) {
         drop();

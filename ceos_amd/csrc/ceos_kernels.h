@@ -28,6 +28,10 @@ struct RowRing {
     int64_t head;
     uint32_t base;   // absolute id of logical id 0
     uint32_t live;   // number of live rows
+    // int8 copies of the rows (same slots, row stride d) and per slot (s, r1, r2, 0):
+    // see k_quantize_rows. Null: every candidate gets the exact rerank.
+    const int8_t* Xq = nullptr;
+    const float4* qsr = nullptr;
 };
 
 // ---------------- build ----------------
@@ -98,6 +102,7 @@ struct QueryConfig {
     int s_max, c_max, k_max;
     int64_t cand_stride;  // scratch entries per workgroup
     int proj_waves = 4;   // query projection: waves per workgroup (2, 4 or 8)
+    int prune = 1;        // rerank: bound every candidate from the int8 rows first (when present)
     // optional kernel timing: recorded before the query projection and before and
     // after the fused search kernel (all null: no events)
     hipEvent_t ev[3];
@@ -143,6 +148,11 @@ hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 
 // Clears bit (slot0 + j) mod cap of a slot bitmap for j < count (count <= cap).
 hipError_t launch_clear_slot_bits(uint32_t* bits, int64_t slot0, int64_t count, int64_t cap, hipStream_t st);
+
+// Rerank bounds for ring slots (slot0 + j) mod cap, j < count: int8 row copies
+// Xq [cap][d] and per slot (s, r1, r2, 0); see k_quantize_rows.
+hipError_t launch_quantize_rows(const float* X, int64_t slot0, int64_t count, int64_t cap, int d, int8_t* Xq,
+                                float4* qsr, hipStream_t st);
 
 // Streaming read of n floats (n % 4 == 0, x 16-byte aligned) by grid x 256 threads, one
 // partial sum per thread into out (grid * 256 floats).

@@ -316,6 +316,15 @@ class PyCEOs {
     }
     int fht_dim() const { int F, D, R, m, seed; check(ceos_get_dims(h_, &F, &D, &R, &m, &seed)); return F; }
     void set_search_config(int wg_per_cu, int table_log2) { check(ceos_set_search_config(h_, wg_per_cu, table_log2)); }
+    bool get_rerank_bounds() const { int v; check(ceos_get_rerank_bounds(h_, &v)); return v != 0; }
+    void set_rerank_bounds(bool v) {
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_set_rerank_bounds(h_, v ? 1 : 0);
+        }
+        check(rc);
+    }
     py::tuple get_search_config() const {
         int wg, t, lds;
         check(ceos_get_search_config(h_, &wg, &t, &lds));
@@ -338,6 +347,15 @@ class PyStream {
     ceos_stream* raw() const { return h_; }
     bool get_centering() const { int v; check(ceos_stream_get_centering(h_, &v)); return v != 0; }
     void set_centering(bool v) { check(ceos_stream_set_centering(h_, v ? 1 : 0)); }
+    bool get_rerank_bounds() const { int v; check(ceos_stream_get_rerank_bounds(h_, &v)); return v != 0; }
+    void set_rerank_bounds(bool v) {
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = ceos_stream_set_rerank_bounds(h_, v ? 1 : 0);
+        }
+        check(rc);
+    }
     int dim() const { return d_; }
 
     void save(const std::string& path) const {
@@ -596,6 +614,7 @@ PYBIND11_MODULE(CEOs, m) {
         .def_property("n_probed_points", &PyCEOs::get_mp, &PyCEOs::set_mp)
         .def_property("n_cand", &PyCEOs::get_c, &PyCEOs::set_c)
         .def_property("centering", &PyCEOs::get_centering, &PyCEOs::set_centering)
+        .def_property("rerank_bounds", &PyCEOs::get_rerank_bounds, &PyCEOs::set_rerank_bounds)
         .def("set_threads", &PyCEOs::set_threads, py::arg("n_threads"))
         .def("clear", &PyCEOs::clear)
         .def("build", &PyCEOs::build, py::arg("dataset"))
@@ -661,6 +680,7 @@ PYBIND11_MODULE(CEOs, m) {
         .def_property("n_probed_points", &PyStream::get_mp, &PyStream::set_mp)
         .def_property("n_cand", &PyStream::get_c, &PyStream::set_c)
         .def_property("centering", &PyStream::get_centering, &PyStream::set_centering)
+        .def_property("rerank_bounds", &PyStream::get_rerank_bounds, &PyStream::set_rerank_bounds)
         .def("set_threads", &PyStream::set_threads, py::arg("n_threads"))
         .def("clear", &PyStream::clear)
         .def("build", &PyStream::build, py::arg("dataset"))

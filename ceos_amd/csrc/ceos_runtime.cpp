@@ -90,6 +90,8 @@ static void build_coceos_impl(ceos_index* h, const float* X, int64_t n, int d, b
     c.load_points(X, n, n, dev, st);
     c.make_signs();
     c.build_buckets(n, st);
+    c.quantize_rows(0, n, n, st);
+    HIP_CHECK(hipStreamSynchronize(st));
     c.built = true;
 }
 
@@ -358,6 +360,33 @@ int ceos_set_search_config(ceos_index* h, int wg_per_cu, int table_log2) {
     });
 }
 
+int ceos_set_rerank_bounds(ceos_index* h, int on) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.set_rerank_bounds(on != 0);
+    });
+}
+int ceos_get_rerank_bounds(const ceos_index* h, int* on) {
+    return guarded([&] {
+        if (!h || !on) throw ArgError("null argument");
+        *on = h->core.rerank_bounds && h->core.d % 4 == 0 ? 1 : 0;
+    });
+}
+int ceos_stream_set_rerank_bounds(ceos_stream* h, int on) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.set_rerank_bounds(on != 0);
+    });
+}
+int ceos_stream_get_rerank_bounds(const ceos_stream* h, int* on) {
+    return guarded([&] {
+        if (!h || !on) throw ArgError("null argument");
+        *on = h->core.rerank_bounds && h->core.d % 4 == 0 ? 1 : 0;
+    });
+}
+
 int ceos_get_search_config(const ceos_index* h, int* wg_per_cu, int* table_log2, int* lds_bytes) {
     return guarded([&] {
         if (!h) throw ArgError("null handle");
@@ -534,6 +563,8 @@ int ceos_stream_build(ceos_stream* h, const float* X, int64_t n, int d) {
         HIP_CHECK(hipMemsetAsync(c.dead.p, 0, c.dead.n * 4, c.st()));
         c.make_signs();
         c.build_buckets(n, c.st());
+        c.quantize_rows(0, n, n, c.st());
+        HIP_CHECK(hipStreamSynchronize(c.st()));
         c.built = true;
     });
 }
@@ -598,6 +629,7 @@ int ceos_stream_update(ceos_stream* h, const float* Xn, int64_t n_new, int d, in
                 HIP_CHECK(hipMemcpyAsync(c.X.p, Xn + seg1 * c.d, (size_t)(n_new - seg1) * c.d * 4,
                                          hipMemcpyHostToDevice, st));
             if (c.dead.p) HIP_CHECK(ceos::launch_clear_slot_bits(c.dead.p, slot0, n_new, c.cap, st));
+            c.quantize_rows(slot0, n_new, n_cur + n_new, st);
         }
         // project the new rows: column-major [DR][n_new], through a row-major projection
         // (coalesced stores) and an LDS-tiled transpose

@@ -25,10 +25,17 @@
 //       the boundary bin, which a radix select resolves exactly; with several passes
 //       the (score, id) keys of all passes are selected from scratch;
 //    c. rerank: FP32 inner products of the candidates' rows with the query, 16 rows
-//       in flight per lane group (8 for the largest bucket depths) (src/CEOs.cpp:638-657);
+//       in flight per lane group (8 for the largest bucket depths) (src/CEOs.cpp:638-657).
+//       With int8 row copies (k_quantize_rows) and k small against c, a bound pass
+//       first reads every candidate's int8 row (a quarter of the bytes) and brackets
+//       its FP32 inner product in [lo, hi]; a candidate whose hi is below the k-th
+//       largest lo cannot reach the top-k, and only the others (typically a little
+//       over k) get the exact FP32 rerank. The top-k is the one reranking every
+//       candidate gives;
 //    d. top-k by (inner product desc, id asc), written descending, zero padded
 //       (src/CEOs.cpp:659-666).
 #include <algorithm>
+#include <cfloat>
 
 #include "ceos_device.h"
 #include "ceos_kernels.h"
@@ -44,12 +51,51 @@ constexpr uint32_t kUnset = 0xffffffffu;       // sum bits of a slot before its 
 typedef const float __attribute__((address_space(1)))* GlobalF;
 typedef float F4v __attribute__((ext_vector_type(4)));
 typedef const F4v __attribute__((address_space(1)))* GlobalF4;
+typedef uint32_t U4v __attribute__((ext_vector_type(4)));
+typedef const U4v __attribute__((address_space(1)))* GlobalU4;
+typedef const uint32_t __attribute__((address_space(1)))* GlobalU1;
 constexpr uint32_t kStashMark = 0xfffffffeu;   // phase-1 slot value: entry sent to the stash
 constexpr uint32_t kToPlace = 0xfffffffdu;     // phase-1 slot value: entry not placed yet
 constexpr int kEmitU = 8;                      // tile slots per thread per emission step
 constexpr int kSelBins = 2048;                 // candidate select: value histogram bins
 constexpr int kBoundMax = 512;                 // boundary-bin keys selected in LDS (else in scratch)
 constexpr int kStashList = kSearchStashList;   // pending stash entries per pass (global list)
+constexpr int kPruneU = 16;                    // bound pass: int8 rows in flight per lane group
+constexpr int kSurvivorMax = kSelBins;         // bound pass: exact-rerank list capacity (hist region)
+
+// Step J of the reduce-scatter of 16 per-lane values over groups of 2^LG lanes: the
+// first min(LG, 4) steps exchange half of the values still carried (the lane whose
+// offset bit is set keeps the upper half), the rest are plain butterflies. Afterwards
+// p[t], t < 16 >> min(LG, 4), is the group sum of value rb + t, where rb sums 8 >> j
+// over the steps j whose offset bit is set in the lane's group index.
+template <int LG, int J>
+__device__ __forceinline__ void reduce_scatter16(float (&p)[16], int gl) {
+    if constexpr (J < LG) {
+        constexpr int O = (1 << LG) >> (J + 1);
+        if constexpr (J < (LG < 4 ? LG : 4)) {
+            constexpr int HALF = 16 >> (J + 1);
+            const bool up = (gl & O) != 0;
+#pragma unroll
+            for (int k = 0; k < HALF; ++k) {
+                const float keep = up ? p[HALF + k] : p[k];
+                const float send = up ? p[k] : p[HALF + k];
+                p[k] = keep + __shfl_xor(send, O, kWave);
+            }
+        } else {
+            p[0] += __shfl_xor(p[0], O, kWave);
+        }
+        reduce_scatter16<LG, J + 1>(p, gl);
+    }
+}
+
+// FP32 sum of four int8 lanes of w (byte j = element j) times q, added to acc.
+__device__ __forceinline__ float dot_i8x4(uint32_t w, float4 q, float acc) {
+    acc = fmaf((float)((int32_t)(w << 24) >> 24), q.x, acc);
+    acc = fmaf((float)((int32_t)(w << 16) >> 24), q.y, acc);
+    acc = fmaf((float)((int32_t)(w << 8) >> 24), q.z, acc);
+    acc = fmaf((float)((int32_t)w >> 24), q.w, acc);
+    return acc;
+}
 
 // One multiplicative hash per id (a single 32-bit multiply) serves both bucket choices
 // (its top bits, then the next bits; 2^lognbk buckets), the claim rotation (bits 4-5)
@@ -761,7 +807,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         // ---- c. rerank ----
         // instantiated once per mode: ids and keys are in LDS (cidl, rk) or in global
         // scratch (idlist, cand), and each copy reads one address space
-        auto rerank = [&](const uint32_t* idsrc, uint64_t* keydst) __attribute__((always_inline)) {
+        auto rerank = [&](const uint32_t* idsrc, uint64_t* keydst, const int nc) __attribute__((always_inline)) {
             const int d = dm.d;
             const bool vec = (d & 3) == 0;
             const int units = vec ? (d >> 2) : d;
@@ -770,7 +816,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             const int groups = kWave / G;
             const int sub = ln / G, gl = ln % G;
             const int per_iter = kSWaves * groups * kRerankU;
-            for (int base = 0; base < ce; base += per_iter) {
+            for (int base = 0; base < nc; base += per_iter) {
                 float acc[kRerankU];
                 GlobalF rp[kRerankU];  // rows are in global memory: plain global loads, never flat
                 uint32_t ids[kRerankU];
@@ -780,7 +826,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     acc[u] = 0.0f;
                     rp[u] = nullptr;
                     ids[u] = 0;
-                    if (i < ce) {
+                    if (i < nc) {
                         ids[u] = idsrc[i];
                         int64_t slot = (int64_t)ids[u] + rows.head;
                         if (slot >= rows.cap) slot -= rows.cap;
@@ -820,15 +866,291 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     float v = acc[u];
                     for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
                     const int i = base + ((u * kSWaves + wave) * groups + sub);
-                    if (i < ce && gl == 0) keydst[i] = ckey(v, ids[u]);
+                    if (i < nc && gl == 0) keydst[i] = ckey(v, ids[u]);
                 }
             }
             __syncthreads();
         };
-        if (hash_mode)
-            rerank(idlist, cand);
-        else
-            rerank(cidl, rk);
+        // Bound pass over the int8 rows: rk[i] becomes the float pair (lo, hi) around
+        // candidate i's FP32 inner product v. Per row, k_quantize_rows stores x ~ s c
+        // (c int8) with r1 = |x - s c| + g |x| and r2 = s |c| (g = (d + 64) 2^-24, the
+        // rounding of any FP32 sum of depth below d + 64).
+        //  * Integer path (d % 16 == 0, 64 <= d <= 1024; one 16-byte unit per lane): the
+        //    query is coded as q^ = t1 b1 + t2 b2 (int8 codes of q and of its residual);
+        //    a = s * fl(sum over lanes of t1 I1 + t2 I2), I = exact int32 dot products of
+        //    the codes with c (v_dot4). |v - a| <= |q| r1 + (|q - q^| + 12u (t1|b1| +
+        //    t2|b2|)) r2, u = 2^-24.
+        //  * Float path (other d): a = s * fl(q . c), |v - a| <= |q| (r1 + g r2).
+        // The margin added to the bound covers the rounding of the bound, lo and hi.
+        auto bound_pass = [&]() __attribute__((always_inline)) {
+            constexpr int U = kPruneU;
+            const int d = dm.d;
+            const bool v16 = (d & 15) == 0;
+            const int units = v16 ? (d >> 4) : (d >> 2);  // 16 or 4 int8 per lane load
+            const bool idot = v16 && units >= 4 && units <= kWave;  // one 16-byte unit per lane
+            int G = 1;
+            if (idot) {
+                while (G < units) G <<= 1;
+            } else {
+                while (2 * G < units && G < kWave) G <<= 1;
+            }
+            const int groups = kWave / G;
+            const int sub = ln / G, gl = ln % G;
+            const int per_iter = kSWaves * groups * U;
+            // |q|_2 rounded up: the FP32 sum of squares is within 2^-12 for d <= 4096
+            float q2 = 0.0f;
+            for (int i = ln; i < d; i += kWave) q2 = fmaf(qv[i], qv[i], q2);
+#pragma unroll
+            for (int o = kWave / 2; o > 0; o >>= 1) q2 += __shfl_xor(q2, o, kWave);
+            const float qn = sqrtf(q2) * (1.0f + 0x1p-10f);
+            float2* lh = reinterpret_cast<float2*>(rk);
+            const float4* q4 = reinterpret_cast<const float4*>(qv);
+            auto slot_of = [&](int i) -> int64_t {
+                int64_t slot = (int64_t)cidl[i] + rows.head;
+                return slot >= rows.cap ? slot - rows.cap : slot;
+            };
+            auto row_params = [&](int64_t slot) -> float4 {
+                const F4v t = ((GlobalF4)rows.qsr)[slot];
+                return make_float4(t.x, t.y, t.z, t.w);
+            };
+            auto emit = [&](int i, float a, float df) {
+                float2 b = make_float2(-INFINITY, INFINITY);
+                if (fabsf(a) <= FLT_MAX && df <= FLT_MAX) {
+                    const float dd = df + (fabsf(a) + df) * 0x1p-20f;
+                    b = make_float2(a - dd, a + dd);
+                }
+                lh[i] = b;
+            };
+            if (idot) {
+                // this lane's 16 query elements (unit gl; idle lanes hold zeros)
+                const bool act = gl < units;
+                float qf[16];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float4 t = act ? q4[4 * gl + k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    qf[4 * k] = t.x; qf[4 * k + 1] = t.y; qf[4 * k + 2] = t.z; qf[4 * k + 3] = t.w;
+                }
+                float m1 = 0.0f;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) m1 = fmaxf(m1, fabsf(qf[e]));
+                for (int o = G >> 1; o > 0; o >>= 1) m1 = fmaxf(m1, __shfl_xor(m1, o, kWave));
+                const bool qfin = m1 <= FLT_MAX;
+                const float t1 = (qfin && m1 > 0.0f) ? m1 / 127.0f : 0.0f;
+                const float i1 = t1 > 0.0f ? 127.0f / m1 : 0.0f;
+                int b1[16];
+                float m2 = 0.0f;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    b1[e] = t1 > 0.0f ? min(127, max(-127, (int)rintf(qf[e] * i1))) : 0;
+                    m2 = fmaxf(m2, fabsf(fmaf(-t1, (float)b1[e], qf[e])));
+                }
+                for (int o = G >> 1; o > 0; o >>= 1) m2 = fmaxf(m2, __shfl_xor(m2, o, kWave));
+                const float t2 = (qfin && m2 > 0.0f) ? m2 / 127.0f : 0.0f;
+                const float i2 = t2 > 0.0f ? 127.0f / m2 : 0.0f;
+                double e2 = 0.0;
+                int n1 = 0, n2 = 0;
+                uint32_t w1[4] = {0u, 0u, 0u, 0u}, w2[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int b2 = t2 > 0.0f ? min(127, max(-127, (int)rintf(fmaf(-t1, (float)b1[e], qf[e]) * i2))) : 0;
+                    const double dv = (double)qf[e] - (double)t1 * (double)b1[e] - (double)t2 * (double)b2;
+                    e2 += dv * dv;
+                    n1 += b1[e] * b1[e];
+                    n2 += b2 * b2;
+                    w1[e >> 2] |= ((uint32_t)b1[e] & 0xffu) << (8 * (e & 3));
+                    w2[e >> 2] |= ((uint32_t)b2 & 0xffu) << (8 * (e & 3));
+                }
+                for (int o = G >> 1; o > 0; o >>= 1) {
+                    e2 += __shfl_xor(e2, o, kWave);
+                    n1 += __shfl_xor(n1, o, kWave);
+                    n2 += __shfl_xor(n2, o, kWave);
+                }
+                float beta = INFINITY;
+                if (qfin) {
+                    const double bb = (sqrt(e2) + 12.0 * 0x1p-24 * ((double)t1 * sqrt((double)n1) +
+                                                                    (double)t2 * sqrt((double)n2))) * (1.0 + 0x1p-20);
+                    beta = (float)bb;
+                    if ((double)beta < bb) beta = nextafterf(beta, INFINITY);
+                }
+                // Lane gl of a G-lane group loads 16 bytes (unit gl) of each of the group's
+                // U rows (idle lanes re-read unit 0: their query codes are zero; rows past
+                // ce re-read row ce - 1). The lane partial sums are then reduced by a
+                // reduce-scatter: each of the first min(LG, 4) xor steps halves the rows a
+                // lane carries, the rest are plain butterflies; a lane ends with the full
+                // sums of NO = 16 >> min(LG, 4) consecutive rows starting at `rb`.
+                static_assert(kPruneU == 16, "the reduce-scatter below sums 16 rows per group");
+                const int lu = act ? gl : 0;
+                auto run = [&](auto lgc) __attribute__((always_inline)) {
+                    constexpr int LG = decltype(lgc)::value;
+                    constexpr int GS = 1 << LG;
+                    constexpr int H = LG < 4 ? LG : 4;
+                    constexpr int NO = U >> H;
+                    int rb = 0;
+#pragma unroll
+                    for (int j = 0; j < H; ++j)
+                        if ((gl >> (LG - 1 - j)) & 1) rb += (U / 2) >> j;
+                    // emitting lanes: one per row copy
+                    const bool emitter = (gl & ((GS >> H) - 1)) == 0;
+                    for (int base = 0; base < ce; base += per_iter) {
+                        uint32_t id[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                            id[u] = cidl[min(base + (u * kSWaves + wave) * groups + sub, ce - 1)];
+                        uint32_t pid[NO];
+#pragma unroll
+                        for (int t = 0; t < NO; ++t)
+                            pid[t] = cidl[min(base + ((rb + t) * kSWaves + wave) * groups + sub, ce - 1)];
+                        U4v w[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            int64_t slot = (int64_t)id[u] + rows.head;
+                            if (slot >= rows.cap) slot -= rows.cap;
+                            w[u] = ((GlobalU4)(rows.Xq + slot * d))[lu];
+                        }
+                        float4 prm[NO];
+#pragma unroll
+                        for (int t = 0; t < NO; ++t) {
+                            int64_t slot = (int64_t)pid[t] + rows.head;
+                            if (slot >= rows.cap) slot -= rows.cap;
+                            prm[t] = row_params(slot);
+                        }
+                        float p[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            int h = 0, l = 0;
+                            h = __builtin_amdgcn_sdot4((int)w[u].x, (int)w1[0], h, false);
+                            l = __builtin_amdgcn_sdot4((int)w[u].x, (int)w2[0], l, false);
+                            h = __builtin_amdgcn_sdot4((int)w[u].y, (int)w1[1], h, false);
+                            l = __builtin_amdgcn_sdot4((int)w[u].y, (int)w2[1], l, false);
+                            h = __builtin_amdgcn_sdot4((int)w[u].z, (int)w1[2], h, false);
+                            l = __builtin_amdgcn_sdot4((int)w[u].z, (int)w2[2], l, false);
+                            h = __builtin_amdgcn_sdot4((int)w[u].w, (int)w1[3], h, false);
+                            l = __builtin_amdgcn_sdot4((int)w[u].w, (int)w2[3], l, false);
+                            p[u] = fmaf(t1, (float)h, t2 * (float)l);
+                        }
+                        reduce_scatter16<LG, 0>(p, gl);
+#pragma unroll
+                        for (int t = 0; t < NO; ++t) {
+                            const int i = base + ((rb + t) * kSWaves + wave) * groups + sub;
+                            if (emitter && i < ce)
+                                emit(i, prm[t].x * p[t], fmaf(qn, prm[t].y, beta * prm[t].z) * (1.0f + 0x1p-20f));
+                        }
+                    }
+                };
+                switch (G) {
+                    case 64: run(std::integral_constant<int, 6>{}); break;
+                    case 32: run(std::integral_constant<int, 5>{}); break;
+                    case 16: run(std::integral_constant<int, 4>{}); break;
+                    case 8: run(std::integral_constant<int, 3>{}); break;
+                    default: run(std::integral_constant<int, 2>{}); break;
+                }
+            } else {
+                constexpr int UF = kPruneU / 2;
+                const int per_iter_f = kSWaves * groups * UF;
+                const float g = (float)(d + 64) * 0x1p-24f * (1.0f + 0x1p-10f);
+                for (int base = 0; base < ce; base += per_iter_f) {
+                    float acc[UF];
+                    const int8_t* rp[UF];
+                    float4 prm[UF];
+#pragma unroll
+                    for (int u = 0; u < UF; ++u) {
+                        const int i = base + ((u * kSWaves + wave) * groups + sub);
+                        acc[u] = 0.0f;
+                        rp[u] = nullptr;
+                        prm[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        if (i < ce) {
+                            const int64_t slot = slot_of(i);
+                            rp[u] = rows.Xq + slot * d;
+                            prm[u] = row_params(slot);
+                        }
+                    }
+                    if (v16) {
+                        for (int uu = gl; uu < units; uu += G) {
+                            U4v w[UF];
+#pragma unroll
+                            for (int u = 0; u < UF; ++u)
+                                w[u] = rp[u] ? ((GlobalU4)rp[u])[uu] : U4v{0u, 0u, 0u, 0u};
+                            const float4 qa = q4[4 * uu], qb4 = q4[4 * uu + 1], qc = q4[4 * uu + 2], qd = q4[4 * uu + 3];
+#pragma unroll
+                            for (int u = 0; u < UF; ++u) {
+                                acc[u] = dot_i8x4(w[u].x, qa, acc[u]);
+                                acc[u] = dot_i8x4(w[u].y, qb4, acc[u]);
+                                acc[u] = dot_i8x4(w[u].z, qc, acc[u]);
+                                acc[u] = dot_i8x4(w[u].w, qd, acc[u]);
+                            }
+                        }
+                    } else {
+                        for (int uu = gl; uu < units; uu += G) {
+                            uint32_t w[UF];
+#pragma unroll
+                            for (int u = 0; u < UF; ++u) w[u] = rp[u] ? ((GlobalU1)rp[u])[uu] : 0u;
+                            const float4 qa = q4[uu];
+#pragma unroll
+                            for (int u = 0; u < UF; ++u) acc[u] = dot_i8x4(w[u], qa, acc[u]);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < UF; ++u) {
+                        float v = acc[u];
+                        for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+                        const int i = base + ((u * kSWaves + wave) * groups + sub);
+                        if (i < ce && gl == 0) emit(i, prm[u].x * v, qn * fmaf(g, prm[u].z, prm[u].y));
+                    }
+                }
+            }
+            __syncthreads();
+        };
+        int nr = ce;  // keys the top-k reads
+        if (hash_mode) {
+            rerank(idlist, cand, ce);
+        } else {
+            const int kk0 = min(kq, ce);
+            const bool prune = rows.Xq != nullptr && cfg.prune != 0 && kk0 > 0 && kk0 <= 256 && 4 * kk0 <= ce &&
+                               (dm.d & 3) == 0;
+            uint32_t* surv = reinterpret_cast<uint32_t*>(lds + lo.hist_off);
+            uint32_t* miscu = reinterpret_cast<uint32_t*>(misc);
+            if (prune) {
+                if (tid == 0) {
+                    miscu[3] = 0xffffffffu;  // smallest lo among the kk largest (fkey)
+                    miscu[11] = 0u;          // survivors
+                }
+                bound_pass();
+                const float2* lh = reinterpret_cast<const float2*>(rk);
+                auto lkey = [&](int i) -> uint64_t { return ((uint64_t)fkey(lh[i].x) << 32) | (uint32_t)i; };
+                uint64_t prefix;
+                int shift;
+                radix_select<kSNT>(ce, kk0, lkey, sh, prefix, shift);
+                for (int i0 = 0; i0 < ce; i0 += kSNT) {
+                    const int i = i0 + tid;
+                    uint32_t m = 0xffffffffu;
+                    if (i < ce) {
+                        const uint64_t k = lkey(i);
+                        if (selected(k, prefix, shift)) m = (uint32_t)(k >> 32);
+                    }
+#pragma unroll
+                    for (int o = kWave / 2; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, kWave));
+                    if (ln == 0 && m != 0xffffffffu) atomicMin(&miscu[3], m);
+                }
+                __syncthreads();
+                const float L = fkey_inv(miscu[3]);
+                for (int i0 = 0; i0 < ce; i0 += kSNT) {
+                    const int i = i0 + tid;
+                    const bool keep = i < ce && lh[i].y >= L;
+                    const uint32_t pos = wave_append(keep, &miscu[11]);
+                    if (keep && pos < (uint32_t)kSurvivorMax) surv[pos] = cidl[i];
+                }
+                __syncthreads();
+                const int ns = (int)miscu[11];
+                if (ns <= kSurvivorMax) {
+                    nr = ns;
+                    rerank(surv, rk, ns);
+                } else {
+                    rerank(cidl, rk, ce);
+                }
+            } else {
+                rerank(cidl, rk, ce);
+            }
+        }
 
         // ---- d. top-k ----
         const int64_t off = qb.out_off ? qb.out_off[q] : (int64_t)q * qb.k0;
@@ -838,12 +1160,12 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             auto topk = [&](auto get) __attribute__((always_inline)) {
                 uint64_t prefix;
                 int shift;
-                radix_select<kSNT>(ce, kk, get, sh, prefix, shift);
+                radix_select<kSNT>(nr, kk, get, sh, prefix, shift);
                 const int N = next_pow2(kk);
                 for (int i = tid; i < N; i += kSNT) tk[i] = 0;
                 if (tid == 0) sh.counter = 0;
                 __syncthreads();
-                compact_selected<kSNT>(ce, get, prefix, shift, &sh.counter,
+                compact_selected<kSNT>(nr, get, prefix, shift, &sh.counter,
                                        [&](uint32_t pos, uint64_t k) { tk[pos] = k; });
                 __syncthreads();
                 bitonic_sort_desc<kSNT>(tk, next_pow2(kk));

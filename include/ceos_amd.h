@@ -119,6 +119,20 @@ int ceos_project(ceos_index* h, const float* rows, int64_t n, int d, float* out)
 int ceos_set_search_config(ceos_index* h, int wg_per_cu, int table_log2);
 int ceos_get_search_config(const ceos_index* h, int* wg_per_cu, int* table_log2, int* lds_bytes);
 
+/* Rerank bounds (no reference counterpart; on by default). While on, a coCEOs or
+ * streamCEOs index keeps an int8 copy of every row (n * d bytes, a quarter of the FP32
+ * rows) with a per-row scale and error bound, built with the index and kept in step by
+ * stream updates. A search whose k is at most 256 and at most a quarter of its
+ * candidates then bounds every candidate's inner product from the int8 row and gives
+ * the exact FP32 rerank only to candidates that can reach the top-k: same results,
+ * about a quarter of the rerank's row reads. Off releases the copies and every
+ * candidate gets the exact rerank. Needs n_features % 4 == 0 (otherwise always off).
+ * Turning it on for a built index makes the copies at once. */
+int ceos_set_rerank_bounds(ceos_index* h, int on);
+int ceos_get_rerank_bounds(const ceos_index* h, int* on);
+int ceos_stream_set_rerank_bounds(ceos_stream* h, int on);
+int ceos_stream_get_rerank_bounds(const ceos_stream* h, int* on);
+
 /* Kernel timing of coCEOs searches (no reference counterpart; for benchmarks). While
  * on, each search launch records HIP events on its stream (up to 4096 launches);
  * turning it on clears earlier records. get_kernel_timing waits for the recorded
