@@ -807,7 +807,13 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         // ---- c. rerank ----
         // instantiated once per mode: ids and keys are in LDS (cidl, rk) or in global
         // scratch (idlist, cand), and each copy reads one address space
-        auto rerank = [&](const uint32_t* idsrc, uint64_t* keydst, const int nc) __attribute__((always_inline)) {
+        // RU rows in flight per lane group; the vector path issues UNR = 16 / RU of a
+        // lane's float4 loads per row together. A row's sum is formed the same way for
+        // every RU: lane gl of its G-lane group adds units gl, gl + G, ... in order, then
+        // the group's butterfly.
+        auto rerank = [&](const uint32_t* idsrc, uint64_t* keydst, const int nc, auto ruc) __attribute__((always_inline)) {
+            constexpr int RU = decltype(ruc)::value;
+            constexpr int UNR = RU >= 16 ? 1 : 16 / RU;
             const int d = dm.d;
             const bool vec = (d & 3) == 0;
             const int units = vec ? (d >> 2) : d;
@@ -815,13 +821,13 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             while (2 * G < units && G < kWave) G <<= 1;
             const int groups = kWave / G;
             const int sub = ln / G, gl = ln % G;
-            const int per_iter = kSWaves * groups * kRerankU;
+            const int per_iter = kSWaves * groups * RU;
             for (int base = 0; base < nc; base += per_iter) {
-                float acc[kRerankU];
-                GlobalF rp[kRerankU];  // rows are in global memory: plain global loads, never flat
-                uint32_t ids[kRerankU];
+                float acc[RU];
+                GlobalF rp[RU];  // rows are in global memory: plain global loads, never flat
+                uint32_t ids[RU];
 #pragma unroll
-                for (int u = 0; u < kRerankU; ++u) {
+                for (int u = 0; u < RU; ++u) {
                     const int i = base + ((u * kSWaves + wave) * groups + sub);
                     acc[u] = 0.0f;
                     rp[u] = nullptr;
@@ -835,34 +841,45 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 }
                 if (vec) {
                     const float4* q4 = reinterpret_cast<const float4*>(qv);
-                    for (int uu = gl; uu < units; uu += G) {
-                        float4 a[kRerankU];
+                    for (int uu0 = gl; uu0 < units; uu0 += UNR * G) {
+                        float4 a[UNR][RU];
 #pragma unroll
-                        for (int u = 0; u < kRerankU; ++u) {
-                            const F4v t = rp[u] ? reinterpret_cast<GlobalF4>(rp[u])[uu] : F4v{0.0f, 0.0f, 0.0f, 0.0f};
-                            a[u] = make_float4(t.x, t.y, t.z, t.w);
+                        for (int v = 0; v < UNR; ++v) {
+                            const int uu = uu0 + v * G;
+#pragma unroll
+                            for (int u = 0; u < RU; ++u) {
+                                const F4v t = (rp[u] && uu < units) ? reinterpret_cast<GlobalF4>(rp[u])[uu]
+                                                                    : F4v{0.0f, 0.0f, 0.0f, 0.0f};
+                                a[v][u] = make_float4(t.x, t.y, t.z, t.w);
+                            }
                         }
-                        const float4 b = q4[uu];
 #pragma unroll
-                        for (int u = 0; u < kRerankU; ++u) {
-                            acc[u] = fmaf(a[u].x, b.x, acc[u]);
-                            acc[u] = fmaf(a[u].y, b.y, acc[u]);
-                            acc[u] = fmaf(a[u].z, b.z, acc[u]);
-                            acc[u] = fmaf(a[u].w, b.w, acc[u]);
+                        for (int v = 0; v < UNR; ++v) {
+                            const int uu = uu0 + v * G;
+                            if (uu < units) {
+                                const float4 b = q4[uu];
+#pragma unroll
+                                for (int u = 0; u < RU; ++u) {
+                                    acc[u] = fmaf(a[v][u].x, b.x, acc[u]);
+                                    acc[u] = fmaf(a[v][u].y, b.y, acc[u]);
+                                    acc[u] = fmaf(a[v][u].z, b.z, acc[u]);
+                                    acc[u] = fmaf(a[v][u].w, b.w, acc[u]);
+                                }
+                            }
                         }
                     }
                 } else {
                     for (int uu = gl; uu < d; uu += G) {
-                        float a[kRerankU];
+                        float a[RU];
 #pragma unroll
-                        for (int u = 0; u < kRerankU; ++u) a[u] = rp[u] ? rp[u][uu] : 0.0f;
+                        for (int u = 0; u < RU; ++u) a[u] = rp[u] ? rp[u][uu] : 0.0f;
                         const float b = qv[uu];
 #pragma unroll
-                        for (int u = 0; u < kRerankU; ++u) acc[u] = fmaf(a[u], b, acc[u]);
+                        for (int u = 0; u < RU; ++u) acc[u] = fmaf(a[u], b, acc[u]);
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < kRerankU; ++u) {
+                for (int u = 0; u < RU; ++u) {
                     float v = acc[u];
                     for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
                     const int i = base + ((u * kSWaves + wave) * groups + sub);
@@ -1102,7 +1119,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         };
         int nr = ce;  // keys the top-k reads
         if (hash_mode) {
-            rerank(idlist, cand, ce);
+            rerank(idlist, cand, ce, std::integral_constant<int, kRerankU>{});
         } else {
             const int kk0 = min(kq, ce);
             const bool prune = rows.Xq != nullptr && cfg.prune != 0 && kk0 > 0 && kk0 <= 256 && 4 * kk0 <= ce &&
@@ -1143,12 +1160,12 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 const int ns = (int)miscu[11];
                 if (ns <= kSurvivorMax) {
                     nr = ns;
-                    rerank(surv, rk, ns);
+                    rerank(surv, rk, ns, std::integral_constant<int, 4>{});
                 } else {
-                    rerank(cidl, rk, ce);
+                    rerank(cidl, rk, ce, std::integral_constant<int, kRerankU>{});
                 }
             } else {
-                rerank(cidl, rk, ce);
+                rerank(cidl, rk, ce, std::integral_constant<int, kRerankU>{});
             }
         }
 
