@@ -67,6 +67,7 @@ def run_mode(args, w, X, Q, dX, dQ, n, d, nq, torch, CEOs):
     s, mp, c = w.s, (w.m if mode == "est" else args.hash_points), w.c
     g = CEOs.CEOs(n, d)
     g.setIndexParam(w.D, w.R, w.m, -1, w.seed)
+    g.rerank_bounds = not args.no_rerank_bounds
     build = g.build if mode == "est" else g.build_CEOs_Hash
     build(X)
     t0 = time.perf_counter()
@@ -108,7 +109,11 @@ def run_mode(args, w, X, Q, dX, dQ, n, d, nq, torch, CEOs):
     if mode == "est":
         per_q = 4.0 * d + 4.0 * s * n + 4.0 * d * min(c, n) + 8 * k
     else:
-        per_q = 4.0 * d + 8.0 * s * mp + 4.0 * d * float(np.mean(distinct)) + 8 * k
+        # with rerank bounds (4 k <= distinct candidates): (d + 16) per candidate plus the
+        # FP32 rows of the k best
+        nd = float(np.mean(distinct))
+        rows_b = (d + 16) * nd + 4.0 * d * k if (g.rerank_bounds and 4 * k <= nd) else 4.0 * d * nd
+        per_q = 4.0 * d + 8.0 * s * mp + rows_b + 8 * k
     achieved = per_q * nq / step_s / 1e9
     out = {
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
@@ -406,6 +411,9 @@ def main():
     pruned = bool(index.rerank_bounds) & (kf >= 1) & (kf <= 256) & (4 * kf <= c_eff)
     rerank_bytes = np.where(pruned, (d + 16) * c_eff + 4 * d * kf, 4 * d * c_eff)
     bytes_launch = float(np.sum(4 * d + 8 * ss.astype(np.float64) * mp + rerank_bytes + 8 * ks))
+    # the byte model of the reference's algorithm (every candidate's FP32 row read), for
+    # comparison with the rate the search would need to match without the bound pass
+    full_bytes = float(np.sum(4 * d + 8 * ss.astype(np.float64) * mp + 4 * d * c_eff + 8 * ks))
     achieved = bytes_launch / (fused_ms / 1e3) / 1e9
     # measured HBM bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel on
     # this workload, scaled by a 1 GiB copy (profiles/k_search_fused_<workload>_traffic.json)
@@ -420,7 +428,9 @@ def main():
             "kernel": "k_search_fused", "kernel_ms": round(fused_ms, 4),
             "bytes_per_query": round(bytes_launch / nq), "front_kernels_ms": round(front_ms, 4),
             "stream_read_gbs": round(read_gbs.value, 1),
-            "frac_of_stream_read": round(achieved / read_gbs.value, 4)}
+            "frac_of_stream_read": round(achieved / read_gbs.value, 4),
+            "full_rerank_bytes_per_launch": round(full_bytes),
+            "full_rerank_equivalent_gbs": round(full_bytes / (fused_ms * 1e-3) / 1e9, 1)}
 
     # ---- build roofline: the FWHT adds of every point (R repeats x 3 rounds x F log2 F) against
     # the non-packed FP32 add peak; HBM bytes = the points read once (4d per point) plus the

@@ -899,7 +899,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         //    t2|b2|)) r2, u = 2^-24.
         //  * Float path (other d): a = s * fl(q . c), |v - a| <= |q| (r1 + g r2).
         // The margin added to the bound covers the rounding of the bound, lo and hi.
-        auto bound_pass = [&]() __attribute__((always_inline)) {
+        auto bound_pass = [&](const uint32_t* idsrc, uint64_t* keydst) __attribute__((always_inline)) {
             constexpr int U = kPruneU;
             const int d = dm.d;
             const bool v16 = (d & 15) == 0;
@@ -920,10 +920,10 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
 #pragma unroll
             for (int o = kWave / 2; o > 0; o >>= 1) q2 += __shfl_xor(q2, o, kWave);
             const float qn = sqrtf(q2) * (1.0f + 0x1p-10f);
-            float2* lh = reinterpret_cast<float2*>(rk);
+            float2* lh = reinterpret_cast<float2*>(keydst);
             const float4* q4 = reinterpret_cast<const float4*>(qv);
             auto slot_of = [&](int i) -> int64_t {
-                int64_t slot = (int64_t)cidl[i] + rows.head;
+                int64_t slot = (int64_t)idsrc[i] + rows.head;
                 return slot >= rows.cap ? slot - rows.cap : slot;
             };
             auto row_params = [&](int64_t slot) -> float4 {
@@ -1012,11 +1012,11 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                         uint32_t id[U];
 #pragma unroll
                         for (int u = 0; u < U; ++u)
-                            id[u] = cidl[min(base + (u * kSWaves + wave) * groups + sub, ce - 1)];
+                            id[u] = idsrc[min(base + (u * kSWaves + wave) * groups + sub, ce - 1)];
                         uint32_t pid[NO];
 #pragma unroll
                         for (int t = 0; t < NO; ++t)
-                            pid[t] = cidl[min(base + ((rb + t) * kSWaves + wave) * groups + sub, ce - 1)];
+                            pid[t] = idsrc[min(base + ((rb + t) * kSWaves + wave) * groups + sub, ce - 1)];
                         U4v w[U];
 #pragma unroll
                         for (int u = 0; u < U; ++u) {
@@ -1117,57 +1117,61 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             }
             __syncthreads();
         };
-        int nr = ce;  // keys the top-k reads
-        if (hash_mode) {
-            rerank(idlist, cand, ce, std::integral_constant<int, kRerankU>{});
-        } else {
+        // Rerank of the ce candidates whose ids are in `ids` into keys `keys` (LDS cidl/rk,
+        // or global idlist/cand in hash mode); nr = keys the top-k reads.
+        int nr = ce;
+        auto finish = [&](const uint32_t* ids, uint64_t* keys) __attribute__((always_inline)) {
             const int kk0 = min(kq, ce);
             const bool prune = rows.Xq != nullptr && cfg.prune != 0 && kk0 > 0 && kk0 <= 256 && 4 * kk0 <= ce &&
                                (dm.d & 3) == 0;
+            if (!prune) {
+                rerank(ids, keys, ce, std::integral_constant<int, kRerankU>{});
+                return;
+            }
             uint32_t* surv = reinterpret_cast<uint32_t*>(lds + lo.hist_off);
             uint32_t* miscu = reinterpret_cast<uint32_t*>(misc);
-            if (prune) {
-                if (tid == 0) {
-                    miscu[3] = 0xffffffffu;  // smallest lo among the kk largest (fkey)
-                    miscu[11] = 0u;          // survivors
-                }
-                bound_pass();
-                const float2* lh = reinterpret_cast<const float2*>(rk);
-                auto lkey = [&](int i) -> uint64_t { return ((uint64_t)fkey(lh[i].x) << 32) | (uint32_t)i; };
-                uint64_t prefix;
-                int shift;
-                radix_select<kSNT>(ce, kk0, lkey, sh, prefix, shift);
-                for (int i0 = 0; i0 < ce; i0 += kSNT) {
-                    const int i = i0 + tid;
-                    uint32_t m = 0xffffffffu;
-                    if (i < ce) {
-                        const uint64_t k = lkey(i);
-                        if (selected(k, prefix, shift)) m = (uint32_t)(k >> 32);
-                    }
-#pragma unroll
-                    for (int o = kWave / 2; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, kWave));
-                    if (ln == 0 && m != 0xffffffffu) atomicMin(&miscu[3], m);
-                }
-                __syncthreads();
-                const float L = fkey_inv(miscu[3]);
-                for (int i0 = 0; i0 < ce; i0 += kSNT) {
-                    const int i = i0 + tid;
-                    const bool keep = i < ce && lh[i].y >= L;
-                    const uint32_t pos = wave_append(keep, &miscu[11]);
-                    if (keep && pos < (uint32_t)kSurvivorMax) surv[pos] = cidl[i];
-                }
-                __syncthreads();
-                const int ns = (int)miscu[11];
-                if (ns <= kSurvivorMax) {
-                    nr = ns;
-                    rerank(surv, rk, ns, std::integral_constant<int, 4>{});
-                } else {
-                    rerank(cidl, rk, ce, std::integral_constant<int, kRerankU>{});
-                }
-            } else {
-                rerank(cidl, rk, ce, std::integral_constant<int, kRerankU>{});
+            if (tid == 0) {
+                miscu[3] = 0xffffffffu;  // smallest lo among the kk largest (fkey)
+                miscu[11] = 0u;          // survivors
             }
-        }
+            bound_pass(ids, keys);
+            const float2* lh = reinterpret_cast<const float2*>(keys);
+            auto lkey = [&](int i) -> uint64_t { return ((uint64_t)fkey(lh[i].x) << 32) | (uint32_t)i; };
+            uint64_t prefix;
+            int shift;
+            radix_select<kSNT>(ce, kk0, lkey, sh, prefix, shift);
+            for (int i0 = 0; i0 < ce; i0 += kSNT) {
+                const int i = i0 + tid;
+                uint32_t m = 0xffffffffu;
+                if (i < ce) {
+                    const uint64_t k = lkey(i);
+                    if (selected(k, prefix, shift)) m = (uint32_t)(k >> 32);
+                }
+#pragma unroll
+                for (int o = kWave / 2; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, kWave));
+                if (ln == 0 && m != 0xffffffffu) atomicMin(&miscu[3], m);
+            }
+            __syncthreads();
+            const float L = fkey_inv(miscu[3]);
+            for (int i0 = 0; i0 < ce; i0 += kSNT) {
+                const int i = i0 + tid;
+                const bool keep = i < ce && lh[i].y >= L;
+                const uint32_t pos = wave_append(keep, &miscu[11]);
+                if (keep && pos < (uint32_t)kSurvivorMax) surv[pos] = ids[i];
+            }
+            __syncthreads();
+            const int ns = (int)miscu[11];
+            if (ns <= kSurvivorMax) {
+                nr = ns;
+                rerank(surv, keys, ns, std::integral_constant<int, 4>{});
+            } else {
+                rerank(ids, keys, ce, std::integral_constant<int, kRerankU>{});
+            }
+        };
+        if (hash_mode)
+            finish(idlist, cand);
+        else
+            finish(cidl, rk);
 
         // ---- d. top-k ----
         const int64_t off = qb.out_off ? qb.out_off[q] : (int64_t)q * qb.k0;

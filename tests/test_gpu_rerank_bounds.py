@@ -27,7 +27,9 @@ def _index(CEOs, X, D=256, R=2, m=400):
 
 def _both(g, Q, k, s, c, search=None):
     """(ids, ips) with the bound pass on and off; the index is left with it on."""
-    g.n_probed_vectors, g.n_cand = s, c
+    g.n_probed_vectors = s
+    if c > 1:
+        g.n_cand = c
     run = search or (lambda: g.search_coCEOs_Est(Q, k))
     assert g.rerank_bounds
     on = run()
@@ -121,3 +123,16 @@ def test_bounds_stream_ring(CEOs):
         g.rerank_bounds = True
         assert np.array_equal(on[0], off[0])
         assert np.array_equal(on[1].view(np.uint32), off[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("d,k", [(128, 10), (960, 10), (300, 1), (64, 100)])
+def test_bounds_hash_mode(CEOs, d, k):
+    """CEOs-Hash reranks every distinct candidate from global scratch: the bound pass
+    there too gives the exact rerank's ids and bits."""
+    X = _rand(15_000, d, 30, sigma=0.7)
+    Q = _rand(48, d, 31, sigma=0.0)
+    g = CEOs.CEOs(*X.shape)
+    g.setIndexParam(256, 2, 300, -1, 42)
+    g.build_CEOs_Hash(X)
+    g.n_probed_vectors, g.n_probed_points = 20, 200
+    _both(g, Q, k, 20, 1, search=lambda: g.search_CEOs_Hash(Q, k))
