@@ -80,3 +80,30 @@ def test_read_bandwidth_argument_checks():
     assert L.ceos_read_bandwidth(p, 1024, p, 100, 1, None, C.byref(g)) == 1         # n_out not a multiple of 256
     assert L.ceos_read_bandwidth(p, 1024, p, 256, 0, None, C.byref(g)) == 1         # iters
     assert b"iters" in L.ceos_last_error()
+
+
+def test_rerank_bounds_toggle_without_device(CEOs):
+    """The rerank-bounds switch is host state until an index is built: on by default,
+    reported off for n_features % 4 != 0 (no int8 copies then), null handles rejected."""
+    L = capi.lib()
+    L.ceos_set_rerank_bounds.argtypes = [C.c_void_p, C.c_int]
+    L.ceos_get_rerank_bounds.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    h = C.c_void_p()
+    v = C.c_int(-1)
+    assert L.ceos_create(100, 16, C.byref(h)) == 0
+    try:
+        assert L.ceos_get_rerank_bounds(h, C.byref(v)) == 0 and v.value == 1
+        assert L.ceos_set_rerank_bounds(h, 0) == 0
+        assert L.ceos_get_rerank_bounds(h, C.byref(v)) == 0 and v.value == 0
+        assert L.ceos_set_rerank_bounds(h, 1) == 0
+        assert L.ceos_get_rerank_bounds(h, C.byref(v)) == 0 and v.value == 1
+        assert L.ceos_get_rerank_bounds(h, None) == 1
+    finally:
+        L.ceos_destroy(h)
+    assert L.ceos_set_rerank_bounds(None, 1) == 1
+    g = CEOs.CEOs(100, 30)
+    assert g.rerank_bounds is False          # 30 % 4 != 0
+    s = CEOs.streamCEOs(32)
+    assert s.rerank_bounds is True
+    s.rerank_bounds = False
+    assert s.rerank_bounds is False
