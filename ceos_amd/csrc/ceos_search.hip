@@ -888,8 +888,9 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             }
             __syncthreads();
         };
-        // Bound pass over the int8 rows: rk[i] becomes the float pair (lo, hi) around
-        // candidate i's FP32 inner product v. Per row, k_quantize_rows stores x ~ s c
+        // Bound pass over the int8 rows: keydst[i] (rk, or cand in hash mode) becomes the
+        // float pair (lo, hi) around candidate i's FP32 inner product v. A non-finite query
+        // or row gives infinite bounds. Per row, k_quantize_rows stores x ~ s c
         // (c int8) with r1 = |x - s c| + g |x| and r2 = s |c| (g = (d + 64) 2^-24, the
         // rounding of any FP32 sum of depth below d + 64).
         //  * Integer path (d % 16 == 0, 64 <= d <= 1024; one 16-byte unit per lane): the
