@@ -150,15 +150,25 @@ int search_lds_bytes(int d, int table_log2, int s_max, int c_max, int k_max) {
 // ---------------------------------------------------------------------------------
 // STAGE: the query's DR projections are copied to LDS once (after the keys) and the radix
 // passes read them there; without it (DR above kProbeStageMax) they read global memory.
+// With DR >= 2048 and 16 s <= DR, a sampled bound comes first: T = the value at rank
+// r = 2 ceil(s * 256 / DR) + 4 among the |p| of 256 strided columns (a radix select over
+// those 256; T keeps the value bits the select resolved, so it is at most that value).
+// One pass gathers every column with |p| >= T (NaN included) into a list of at most
+// kProbeList keys, and the exact top-s is selected from the list: the columns left out
+// all rank below every gathered one, so with at least s gathered the result is the exact
+// top-s. Otherwise (a rare tail, or a list overflow) the select runs over all DR columns.
 constexpr int kProbeStageMax = 12288;
+constexpr int kProbeList = 640;  // with DR = 8192 staged, 4 workgroups still fit a CU
 template <bool STAGE>
 __global__ __launch_bounds__(kPNT) void k_probe_select(int DR, const float* __restrict__ qproj, int nq,
                                                        const int32_t* __restrict__ nq_dev,
                                                        const int32_t* __restrict__ s_arr, int s0, int s_stride,
                                                        int32_t* __restrict__ probes) {
     __shared__ SelectShared sh;
-    extern __shared__ uint64_t pkey[];  // next_pow2(s_stride) keys (at least 2), then (STAGE) the DR projections
-    float* pl = reinterpret_cast<float*>(pkey + max(2, next_pow2(s_stride)));  // 16-byte aligned
+    // next_pow2(s_stride) keys (at least 2), the gathered list, then (STAGE) the DR projections
+    extern __shared__ uint64_t pkey[];
+    uint64_t* plist = pkey + max(2, next_pow2(s_stride));
+    float* pl = reinterpret_cast<float*>(plist + kProbeList);  // 16-byte aligned
     if (nq_dev) nq = min(nq, nq_dev[0]);
     for (int q = blockIdx.x; q < nq; q += gridDim.x) {
         const int sq = min(max(s_arr ? s_arr[q] : s0, 0), min(s_stride, DR));
@@ -192,15 +202,42 @@ __global__ __launch_bounds__(kPNT) void k_probe_select(int DR, const float* __re
         }
         const float* src = STAGE ? pl : pq;
         auto get = [&](int c) -> uint64_t { return ckey(fabsf(src[c]), (uint32_t)c); };
-        uint64_t prefix;
-        int shift;
-        radix_select<kPNT>(DR, sq, get, sh, prefix, shift);
+        int nlist = -1;  // keys gathered above the sampled bound; -1: select over all columns
+        if (DR >= 8 * kPNT && 16 * sq <= DR) {  // uniform
+            const int stride = DR / kPNT;
+            const int r = 2 * ((sq * kPNT + DR - 1) / DR) + 4;
+            uint64_t sp;
+            int ssh;
+            radix_select<kPNT>(kPNT, r, [&](int i) -> uint64_t { return get(i * stride); }, sh, sp, ssh);
+            const float T = fkey_inv((uint32_t)(sp >> 32));
+            if (threadIdx.x == 0) sh.counter = 0;
+            __syncthreads();
+            for (int c0 = 0; c0 < DR; c0 += kPNT) {
+                const int c = c0 + (int)threadIdx.x;
+                const bool take = c < DR && !(fabsf(src[c]) < T);
+                const uint32_t pos = wave_append(take, &sh.counter);
+                if (take && pos < (uint32_t)kProbeList) plist[pos] = get(c);
+            }
+            __syncthreads();
+            const int cnt = (int)sh.counter;
+            nlist = (cnt >= sq && cnt <= kProbeList) ? cnt : -1;
+            __syncthreads();  // sh is reused below
+        }
         const int N = next_pow2(sq);
-        for (int i = threadIdx.x; i < N; i += kPNT) pkey[i] = 0;
-        if (threadIdx.x == 0) sh.counter = 0;
-        __syncthreads();
-        compact_selected<kPNT>(DR, get, prefix, shift, &sh.counter, [&](uint32_t pos, uint64_t k) { pkey[pos] = k; });
-        __syncthreads();
+        auto select_top = [&](int n, auto g) __attribute__((always_inline)) {
+            uint64_t prefix;
+            int shift;
+            radix_select<kPNT>(n, sq, g, sh, prefix, shift);
+            for (int i = threadIdx.x; i < N; i += kPNT) pkey[i] = 0;
+            if (threadIdx.x == 0) sh.counter = 0;
+            __syncthreads();
+            compact_selected<kPNT>(n, g, prefix, shift, &sh.counter, [&](uint32_t pos, uint64_t k) { pkey[pos] = k; });
+            __syncthreads();
+        };
+        if (nlist >= 0)
+            select_top(nlist, [&](int i) -> uint64_t { return plist[i]; });
+        else
+            select_top(DR, get);
         bitonic_sort_desc<kPNT>(pkey, N);
         for (int j = threadIdx.x; j < sq; j += kPNT) {
             const int c = (int)ckey_id(pkey[sq - 1 - j]);
@@ -1225,7 +1262,7 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
     if (err != hipSuccess) return err;
     {
         const int pgrid = std::min(qb.nq, 4096);
-        const size_t klds = (size_t)std::max(2, next_pow2(cfg.s_max)) * 8;
+        const size_t klds = ((size_t)std::max(2, next_pow2(cfg.s_max)) + kProbeList) * 8;
         if (DR <= kProbeStageMax) {
             const size_t plds = klds + (size_t)DR * 4;
             allow_lds_s(k_probe_select<true>, plds);

@@ -190,3 +190,20 @@ def test_ceos_est_sampled_bound(CEOs, oracle, n_points):
     gi, gp = g.search(Q, 10)
     oi, op = o.search(Q, 10)
     _compare_search(gi, gp, oi, op, tol_frac=0.0)
+
+
+@pytest.mark.parametrize("s", [8, 64, 256])
+def test_probe_select_sampled_bound(CEOs, oracle, s):
+    """D*R = 4096 projections per query: the probe select gathers the columns above a
+    bound sampled from 256 strided |p| values and selects the top-s from that list
+    (16 s <= D*R); s = 256 is also the largest s that takes the sampled path. The
+    probe order decides the summation order, so ids must equal the oracle's."""
+    X = _rand(20_000, 64, 41)
+    Q = _rand(120, 64, 42)
+    g, o = _build_pair(CEOs, oracle, X, 1024, 4, 100)
+    for obj in (g, o):
+        obj.n_probed_vectors = s
+        obj.n_cand = 300
+    gi, gp = g.search_coCEOs_Est(Q, 10)
+    oi, op = o.search_coCEOs_Est(Q, 10)
+    _compare_search(gi, gp, oi, op, tol_frac=0.0)
