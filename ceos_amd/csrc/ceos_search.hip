@@ -1168,29 +1168,17 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             }
             uint32_t* surv = reinterpret_cast<uint32_t*>(lds + lo.hist_off);
             uint32_t* miscu = reinterpret_cast<uint32_t*>(misc);
-            if (tid == 0) {
-                miscu[3] = 0xffffffffu;  // smallest lo among the kk largest (fkey)
-                miscu[11] = 0u;          // survivors
-            }
+            if (tid == 0) miscu[11] = 0u;  // survivors
             bound_pass(ids, keys);
             const float2* lh = reinterpret_cast<const float2*>(keys);
             auto lkey = [&](int i) -> uint64_t { return ((uint64_t)fkey(lh[i].x) << 32) | (uint32_t)i; };
             uint64_t prefix;
             int shift;
             radix_select<kSNT>(ce, kk0, lkey, sh, prefix, shift);
-            for (int i0 = 0; i0 < ce; i0 += kSNT) {
-                const int i = i0 + tid;
-                uint32_t m = 0xffffffffu;
-                if (i < ce) {
-                    const uint64_t k = lkey(i);
-                    if (selected(k, prefix, shift)) m = (uint32_t)(k >> 32);
-                }
-#pragma unroll
-                for (int o = kWave / 2; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o, kWave));
-                if (ln == 0 && m != 0xffffffffu) atomicMin(&miscu[3], m);
-            }
-            __syncthreads();
-            const float L = fkey_inv(miscu[3]);
+            // L: the value bits of the select's prefix. Every selected key is at least the
+            // prefix (its bits below `shift` are zero), so L is at most the kk-th largest lo,
+            // and equal to it when the select resolved all 32 value bits (shift < 32).
+            const float L = shift >= 63 ? -INFINITY : fkey_inv((uint32_t)(prefix >> 32));
             for (int i0 = 0; i0 < ce; i0 += kSNT) {
                 const int i = i0 + tid;
                 const bool keep = i < ce && lh[i].y >= L;
