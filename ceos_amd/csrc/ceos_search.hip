@@ -325,7 +325,9 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         }
         __syncthreads();
         const int E = misc[1];
-        uint32_t P = (uint32_t)max(1, (int)ceilf((float)E * fminf(1.0f, frac * 1.25f) / (0.7f * (float)T)));
+        // hash partitions: the expected distinct ids of a pass fill at most 90% of the tile
+        // (an overflow restarts the pass with twice the partitions)
+        uint32_t P = (uint32_t)max(1, (int)ceilf((float)E * fminf(1.0f, frac) / (0.9f * (float)T)));
         const int mp = qb.mp;
         const int jlast = sq - 1;
 
