@@ -276,6 +276,7 @@ def main():
     fused_ms = fused.value / max(1, nl.value)
     front_ms = front.value / max(1, nl.value)
     log(f"kernel events: fused search {fused_ms:.3f} ms/launch, projection+probe select {front_ms:.3f} ms/launch")
+    _, tile_log2, tile_lds = index.get_search_config()
     index.set_search_config(0, 0)
 
     # ---- the practical streaming-read rate of this GPU: a plain grid-stride 16-byte read
@@ -471,7 +472,9 @@ def main():
                    "n_probed_vectors": f"per query U[{ss.min()},{s_max}]" if uneven else w.s,
                    "n_cand": f"per query U[{cs.min()},{c_max}]" if uneven else w.c,
                    "batch": "all queries in one hipGraph-captured plan (projection, probe select, fused search)",
-                   "launch": {"wg_per_cu": wg, "table_log2": tl, "proj_waves_per_wg": plan.proj_waves()}},
+                   "launch": {"wg_per_cu": wg, "table_log2": tl, "tile": "automatic (most buckets that fit)" if tl == 0
+                              else "2^table_log2 slots", "tile_log2_floor": tile_log2, "lds_bytes": tile_lds,
+                              "proj_waves_per_wg": plan.proj_waves()}},
         "recall_at_10": round(rec10, 4) if rec10 is not None else None,
         "recall_at_k": round(rec_k, 4),
         "plain_qps": round(plain_qps, 1) if plain_qps else None,

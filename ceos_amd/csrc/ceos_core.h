@@ -154,7 +154,7 @@ struct ScratchSet {
         cand.ensure((size_t)cfg.grid * cfg.cand_stride);
         pairs.ensure((size_t)cfg.grid * cfg.cand_stride);
         stash.ensure((size_t)cfg.grid * ceos::kSearchStashList);
-        tkeys.ensure((size_t)cfg.grid * ceos::search_tile_slots(cfg.table_log2));
+        tkeys.ensure((size_t)cfg.grid * ceos::search_tile_slots_nbk(cfg.tile_buckets));
         qproj.ensure((size_t)std::max(nq, 1) * DR);
         probes.ensure((size_t)std::max(nq, 1) * cfg.s_max);
         if (!work.p) work.alloc(1);
@@ -393,15 +393,19 @@ struct Core {
         c_max = (int)std::min<int64_t>(c_max, (int64_t)s_max * std::max(1, std::min(mp, m)));
         const int wg = wg_req > 0 ? wg_req : 1;
         const int budget = std::min(kMaxLdsBytes, (160 * 1024) / wg - 2048);
-        int t = 0;
+        // explicit: 2^tl_req slots; automatic: the most buckets (a multiple of 64, at
+        // least 64, at most 2^13) that fit the budget
+        int nbk = 0;
         if (tl_req > 0) {
-            t = tl_req;
+            nbk = 1 << (tl_req - 2);
         } else {
-            for (int cand_t = 15; cand_t >= 8; --cand_t)
-                if (ceos::search_lds_bytes(d, cand_t, s_max, c_max, k_max) <= budget) { t = cand_t; break; }
-            if (t == 0) t = 8;
+            for (int cand = 8192; cand >= 64; cand -= 64)
+                if (ceos::search_lds_bytes(d, cand, s_max, c_max, k_max) <= budget) { nbk = cand; break; }
+            if (nbk == 0) nbk = 64;
         }
-        const int lds = ceos::search_lds_bytes(d, t, s_max, c_max, k_max);
+        int t = 0;  // floor(log2(4 nbk))
+        while ((2 << t) <= 4 * nbk) ++t;
+        const int lds = ceos::search_lds_bytes(d, nbk, s_max, c_max, k_max);
         if (lds > kMaxLdsBytes)
             throw ArgError("search parameters need " + std::to_string(lds) +
                            " bytes of LDS per workgroup (n_cand or n_probed_vectors too large)");
@@ -411,6 +415,7 @@ struct Core {
             throw ArgError("n_probed_vectors * n_probed_points above 2^26 is not supported");
         ceos::QueryConfig cfg{};
         cfg.table_log2 = t;
+        cfg.tile_buckets = nbk;
         cfg.lds_bytes = lds;
         const int fit = std::max(1, (160 * 1024) / (lds + 2048));
         cfg.grid = device_cus() * std::min(wg, fit);
@@ -757,12 +762,15 @@ struct ceos_plan {
     // so the batch's outputs are valid afterwards.
     void tune(int nq, hipStream_t st) {
         Core& x = *core;
-        const int t1 = x.search_config_with(s_max, c_max, k_max, 1, 0).table_log2;
-        const int t2 = x.search_config_with(s_max, c_max, k_max, 2, 0).table_log2;
+        // automatic tiles (the most buckets that fit) and the power of two below each
+        const ceos::QueryConfig c1 = x.search_config_with(s_max, c_max, k_max, 1, 0);
+        const ceos::QueryConfig c2 = x.search_config_with(s_max, c_max, k_max, 2, 0);
         struct Shape { int wg, tl, pw; };
         std::vector<Shape> cands = {{1, 0, 4}, {2, 0, 4}};
-        if (t1 - 1 >= 8) cands.push_back({1, t1 - 1, 4});
-        if (t2 - 1 >= 8) cands.push_back({2, t2 - 1, 4});
+        const int p1 = (1 << c1.table_log2) == 4 * c1.tile_buckets ? c1.table_log2 - 1 : c1.table_log2;
+        const int p2 = (1 << c2.table_log2) == 4 * c2.tile_buckets ? c2.table_log2 - 1 : c2.table_log2;
+        if (p1 >= 8) cands.push_back({1, p1, 4});
+        if (p2 >= 8) cands.push_back({2, p2, 4});
         hipEvent_t e0, e1;
         HIP_CHECK(hipEventCreate(&e0));
         HIP_CHECK(hipEventCreate(&e1));

@@ -97,7 +97,8 @@ struct QueryBatch {
 
 struct QueryConfig {
     int grid;          // persistent workgroups
-    int table_log2;    // LDS hash table slots = 2^table_log2
+    int table_log2;    // log2 of the hashed slots, rounded down (reporting; explicit sizes)
+    int tile_buckets;  // hashed slots = 4 * tile_buckets (two-choice buckets of four keys)
     int lds_bytes;     // dynamic LDS per workgroup
     int s_max, c_max, k_max;
     int64_t cand_stride;  // scratch entries per workgroup
@@ -114,7 +115,7 @@ struct QueryScratch {
     uint64_t* cand;    // grid * cand_stride candidate keys
     uint2* pairs;      // grid * cand_stride (slot, value) per probed bucket position
     uint4* stash;      // grid * kSearchStashList entries that missed both tile buckets
-    uint32_t* keys;    // grid * search_tile_slots(table_log2) tile ids during the ordered pass
+    uint32_t* keys;    // grid * search_tile_slots_nbk(tile_buckets) tile ids during the ordered pass
     int32_t* work;     // work counter (1 int)
     int32_t* status;   // error flags (1 int): bit0 = hash table overflow
 };
@@ -125,13 +126,17 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
                                 const int32_t* bsize, RowRing rows, const QueryBatch& qb,
                                 const QueryConfig& cfg, const QueryScratch& scr, hipStream_t st);
 
-int search_lds_bytes(int d, int table_log2, int s_max, int c_max, int k_max);
-// Score-tile slots: 2^table_log2 hashed slots plus a linear-probing overflow region.
+int search_lds_bytes(int d, int tile_buckets, int s_max, int c_max, int k_max);
+// Score tile: 4 * nbk hashed slots (nbk two-choice buckets of four keys) plus a
+// linear-probing overflow region, a power of two near 1/16 of them, at most 2048.
 constexpr int kSearchStashSlots = 2048;
-__host__ __device__ inline int search_tile_slots(int table_log2) {
-    const int t = 1 << table_log2;
-    return t + (t / 16 < kSearchStashSlots ? t / 16 : kSearchStashSlots);
+__host__ __device__ inline int search_stash_slots(int nbk) {
+    int s = 1;
+    while (2 * s <= nbk / 4 && 2 * s <= kSearchStashSlots) s *= 2;
+    return s;
 }
+__host__ __device__ inline int search_tile_slots_nbk(int nbk) { return 4 * nbk + search_stash_slots(nbk); }
+__host__ __device__ inline int search_tile_slots(int table_log2) { return search_tile_slots_nbk(1 << (table_log2 - 2)); }
 int search_ept(int mp);
 constexpr int kSearchThreads = 512;
 constexpr int kMaxProbedPoints = 4096;

@@ -821,7 +821,7 @@ int ceos_plan_config(const ceos_plan* p, int* wg_per_cu, int* table_log2, double
     return guarded([&] {
         if (!p) throw ArgError("null plan");
         if (wg_per_cu) *wg_per_cu = p->cfg.grid > 0 ? p->cfg.grid / std::max(1, device_cus()) : 0;
-        if (table_log2) *table_log2 = p->cfg.table_log2;
+        if (table_log2) *table_log2 = p->tl;  // 0: the automatic tile (most buckets that fit)
         if (tuned_ms) *tuned_ms = p->tuned ? p->tuned_ms : 0.0;
     });
 }

@@ -7,7 +7,7 @@
 //    list ids 2c (close, p >= 0) / 2c+1 (far, p < 0) (src/CEOs.cpp:517-548, 562-564).
 // 3. k_search_fused (persistent, one 512-thread workgroup per query at a time, next
 //    query from a global work counter so uneven k / s / c balance across the chip):
-//    a. score accumulation in two phases over an LDS tile of 2^table_log2 hashed slots
+//    a. score accumulation in two phases over an LDS tile of 4 * tile_buckets hashed slots
 //       plus a small overflow region.
 //       Phase 1 gives every distinct id of the probed lists a slot (two-choice
 //       buckets of four 4-byte ids, claims by atomicMin between barriers, a small
@@ -98,13 +98,15 @@ __device__ __forceinline__ float dot_i8x4(uint32_t w, float4 q, float acc) {
 }
 
 // One multiplicative hash per id (a single 32-bit multiply) serves both bucket choices
-// (its top bits, then the next bits; 2^lognbk buckets), the claim rotation (bits 4-5)
-// and the pass partition (high half folded onto the low half).
+// (multiply-high of the hash, and of the hash rotated by 16 bits, by the bucket count nbk),
+// the claim rotation (bits 4-5) and the pass partition (high half folded onto the low half).
 __device__ __forceinline__ uint32_t idhash(uint32_t id) { return id * 0x9e3779b1u; }
-__device__ __forceinline__ uint32_t bucket1(uint32_t h, int lognbk) { return h >> (32 - lognbk); }
-__device__ __forceinline__ uint32_t bucket2(uint32_t h, uint32_t b1, int lognbk) {
-    const uint32_t b2 = (h >> (32 - 2 * lognbk)) & ((1u << lognbk) - 1u);
-    return b2 == b1 ? b1 ^ 1u : b2;
+__device__ __forceinline__ uint32_t bucket1(uint32_t h, uint32_t nbk) {
+    return (uint32_t)(((uint64_t)h * nbk) >> 32);
+}
+__device__ __forceinline__ uint32_t bucket2(uint32_t h, uint32_t b1, uint32_t nbk) {
+    const uint32_t b2 = (uint32_t)(((uint64_t)((h << 16) | (h >> 16)) * nbk) >> 32);
+    return b2 != b1 ? b2 : (b1 + 1u == nbk ? 0u : b1 + 1u);
 }
 __device__ __forceinline__ uint32_t pass_of(uint32_t h, uint32_t P) {
     return (((h ^ (h >> 15)) & 0xffffu) * P) >> 16;
@@ -124,7 +126,7 @@ struct SLayout {
     int tab_off, rk_off, tk_off, total;
 };
 
-__host__ __device__ inline SLayout s_layout(int d, int tlog2, int s_max, int c_max, int k_max) {
+__host__ __device__ inline SLayout s_layout(int d, int nbk, int s_max, int c_max, int k_max) {
     SLayout o;
     int off = 0;
     o.q_off = off; off += al16(d * 4);
@@ -135,7 +137,7 @@ __host__ __device__ inline SLayout s_layout(int d, int tlog2, int s_max, int c_m
     o.hist_off = off; off += kSelBins * 4;
     o.u_off = off;
     o.tab_off = off;
-    const int endA = off + 4 * search_tile_slots(tlog2);
+    const int endA = off + 4 * search_tile_slots_nbk(nbk);
     o.rk_off = off;
     o.tk_off = off + al16(c_max * 8);
     const int endB = o.tk_off + next_pow2(k_max) * 8;
@@ -143,8 +145,8 @@ __host__ __device__ inline SLayout s_layout(int d, int tlog2, int s_max, int c_m
     return o;
 }
 
-int search_lds_bytes(int d, int table_log2, int s_max, int c_max, int k_max) {
-    return s_layout(d, table_log2, s_max, c_max, k_max).total;
+int search_lds_bytes(int d, int tile_buckets, int s_max, int c_max, int k_max) {
+    return s_layout(d, tile_buckets, s_max, c_max, k_max).total;
 }
 
 // ---------------------------------------------------------------------------------
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
     constexpr int B = kBufEntries;         // entries per thread per chunk
     __shared__ SelectShared sh;
     extern __shared__ __align__(16) unsigned char lds[];
-    const SLayout lo = s_layout(dm.d, cfg.table_log2, cfg.s_max, cfg.c_max, cfg.k_max);
+    const SLayout lo = s_layout(dm.d, cfg.tile_buckets, cfg.s_max, cfg.c_max, cfg.k_max);
     float* qv = reinterpret_cast<float*>(lds + lo.q_off);
     int* probes = reinterpret_cast<int*>(lds + lo.probe_off);
     int* lims = reinterpret_cast<int*>(lds + lo.lim_off);
@@ -274,13 +276,13 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
     uint64_t* tile64 = reinterpret_cast<uint64_t*>(lds + lo.tab_off);
     uint64_t* rk = reinterpret_cast<uint64_t*>(lds + lo.rk_off);
     uint64_t* tk = reinterpret_cast<uint64_t*>(lds + lo.tk_off);
-    const int Tmain = 1 << cfg.table_log2;         // hashed slots: 2^lognbk buckets of 4 keys (16 bytes)
-    const int lognbk = cfg.table_log2 - 2;
-    const int T = search_tile_slots(cfg.table_log2);  // plus the overflow (stash) region at the end
+    const uint32_t nbk = (uint32_t)cfg.tile_buckets;  // hashed slots: nbk buckets of 4 keys (16 bytes)
+    const int Tmain = 4 * (int)nbk;
+    const int T = search_tile_slots_nbk((int)nbk);    // plus the overflow (stash) region at the end
     const int sslots = T - Tmain;
     uint64_t* cand = scr.cand + (int64_t)blockIdx.x * cfg.cand_stride;
     uint2* pairs = scr.pairs + (int64_t)blockIdx.x * cfg.cand_stride;
-    uint32_t* keys_g = scr.keys + (int64_t)blockIdx.x * search_tile_slots(cfg.table_log2);
+    uint32_t* keys_g = scr.keys + (int64_t)blockIdx.x * T;
     uint4* stash_list = scr.stash + (int64_t)blockIdx.x * kStashList;
     const int tid = threadIdx.x, wave = tid / kWave, ln = tid % kWave;
     float frac = 0.5f;  // running estimate of distinct ids / entries for this workgroup
@@ -373,8 +375,8 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 inpos |= (pos_ok ? 1u : 0u) << b;
                 const bool valid = pos_ok && (P == 1 || pass_of(hh[b], P) == pass);
                 slot[b] = valid ? kToPlace : kEmptyKey;  // kEmptyKey: not in this pass
-                b1[b] = bucket1(hh[b], lognbk);
-                b2[b] = bucket2(hh[b], b1[b], lognbk);
+                b1[b] = bucket1(hh[b], nbk);
+                b2[b] = bucket2(hh[b], b1[b], nbk);
             }
             uint4 k1[B], k2[B];
 #pragma unroll
@@ -445,7 +447,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 const uint4 e = stash_list[i];
                 const uint32_t idv = e.x;
                 const uint32_t hv = idhash(idv);
-                const uint32_t b1 = bucket1(hv, lognbk), b2 = bucket2(hv, b1, lognbk);
+                const uint32_t b1 = bucket1(hv, nbk), b2 = bucket2(hv, b1, nbk);
                 const uint4 p = reinterpret_cast<const uint4*>(keys)[b1];
                 const uint4 q2 = reinterpret_cast<const uint4*>(keys)[b2];
                 const uint32_t m1 = (p.x == idv ? 1u : 0u) | (p.y == idv ? 2u : 0u) | (p.z == idv ? 4u : 0u) |

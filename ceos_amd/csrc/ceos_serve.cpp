@@ -339,7 +339,7 @@ int64_t plan_bytes(const Core& x, const ceos_server_config& c, const ceos::Query
     int64_t b = (int64_t)c.max_batch * (x.d * 4 + 3 * 4 + (int64_t)c.k_max * 8);
     b += (int64_t)c.max_batch * (x.DR() * 4 + (int64_t)qc.s_max * 4);
     b += (int64_t)qc.grid * (qc.cand_stride * 16 + (int64_t)ceos::kSearchStashList * 16 +
-                             4ll * ceos::search_tile_slots(qc.table_log2));
+                             4ll * ceos::search_tile_slots_nbk(qc.tile_buckets));
     return b;
 }
 

@@ -115,7 +115,9 @@ int ceos_get_sign_bits(const ceos_index* h, uint8_t* bits);
 int ceos_project(ceos_index* h, const float* rows, int64_t n, int d, float* out);
 
 /* Tuning of the search launch (no reference counterpart): workgroups per CU and log2
- * of the LDS score-tile slots; 0 = automatic. */
+ * of the LDS score-tile slots; 0 = automatic (for the tile: the most hashed slots, a
+ * multiple of 256, that fit the workgroup's LDS share). get returns the last launch's
+ * slots as floor(log2) and its LDS bytes. */
 int ceos_set_search_config(ceos_index* h, int wg_per_cu, int table_log2);
 int ceos_get_search_config(const ceos_index* h, int* wg_per_cu, int* table_log2, int* lds_bytes);
 
@@ -183,7 +185,8 @@ int ceos_plan_run(ceos_plan* p, int nq, void* stream);
  * nq x k_max) and synchronises. */
 int ceos_plan_search(ceos_plan* p, const float* Q, int nq, int d, const int32_t* k, const int32_t* s,
                      const int32_t* c, int32_t* ids, float* ips);
-/* Launch shape in use (after tuning) and the tuned kernel time in ms (0 if untuned). */
+/* Launch shape in use (after tuning; table_log2 0 = the automatic tile) and the tuned
+ * kernel time in ms (0 if untuned). */
 int ceos_plan_config(const ceos_plan* p, int* wg_per_cu, int* table_log2, double* tuned_ms);
 /* The query projection's waves per workgroup in use (2, 4 or 8; tuned with the rest). */
 int ceos_plan_proj_waves(const ceos_plan* p, int* waves_per_wg);

@@ -207,3 +207,20 @@ def test_probe_select_sampled_bound(CEOs, oracle, s):
     gi, gp = g.search_coCEOs_Est(Q, 10)
     oi, op = o.search_coCEOs_Est(Q, 10)
     _compare_search(gi, gp, oi, op, tol_frac=0.0)
+
+
+@pytest.mark.parametrize("wg,tl", [(1, 0), (2, 0), (1, 12), (1, 9)])
+def test_search_tile_sizes(CEOs, oracle, wg, tl):
+    """The automatic tile takes the most hashed slots that fit the LDS share (not a
+    power of two in general: buckets are chosen by multiply-high of the id hash); small
+    explicit tiles force several hash partitions. Every shape gives the oracle's ids."""
+    X = _rand(60_000, 96, 51)
+    Q = _rand(150, 96, 52)
+    g, o = _build_pair(CEOs, oracle, X, 256, 2, 1500)
+    g.set_search_config(wg, tl)
+    for obj in (g, o):
+        obj.n_probed_vectors = 60
+        obj.n_cand = 2000
+    gi, gp = g.search_coCEOs_Est(Q, 10)
+    oi, op = o.search_coCEOs_Est(Q, 10)
+    _compare_search(gi, gp, oi, op, tol_frac=0.0)

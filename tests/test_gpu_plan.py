@@ -71,7 +71,7 @@ def test_plan_autotune_keeps_results(index):
     I0, P0 = _rows(ids, ips, off, len(Q), 100)
     pi, pp = plan.search(Q, k, s, c)  # first run tunes
     wg, tl, ms = plan.config()
-    assert ms > 0 and wg >= 1 and 8 <= tl <= 15
+    assert ms > 0 and wg >= 1 and (tl == 0 or 8 <= tl <= 15)  # 0: the automatic tile
     assert np.array_equal(pi, I0) and np.array_equal(pp, P0)
     pi, pp = plan.search(Q, k, s, c)
     assert np.array_equal(pi, I0) and np.array_equal(pp, P0)
