@@ -659,21 +659,25 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         // loads in flight together, and the keys go to the tile (when they fit) or stay in
         // scratch. Returns the number of keys, or -1.
         auto sampled_compaction = [&]() __attribute__((always_inline)) -> int {
-            uint64_t* skey = reinterpret_cast<uint64_t*>(lds + lo.hist_off);  // kSNT sample keys
-            const int stepS = T / kSNT;
-            const uint32_t sb0 = keys[tid * stepS];
+            // 2 * kSNT strided sample keys (the hist region holds 1024 keys)
+            constexpr int NS = 2 * kSNT;
+            static_assert(NS * 8 <= kSelBins * 4, "sample keys fit the histogram region");
+            uint64_t* skey = reinterpret_cast<uint64_t*>(lds + lo.hist_off);
+            const int stepS = T / NS;
+            const uint32_t sb0 = keys[tid * stepS], sb1 = keys[(tid + kSNT) * stepS];
             skey[tid] = sb0 != kUnset ? (uint64_t)fkey(__uint_as_float(sb0)) << 32 : 0ull;
+            skey[tid + kSNT] = sb1 != kUnset ? (uint64_t)fkey(__uint_as_float(sb1)) << 32 : 0ull;
             if (tid == 0) { misc[14] = 0; misc[15] = 0; }
             __syncthreads();
-            const uint64_t occm = __ballot(sb0 != kUnset);
-            if (ln == 0) atomicAdd(reinterpret_cast<uint32_t*>(&misc[14]), (uint32_t)__popcll(occm));
+            const uint32_t occ = (uint32_t)__popcll(__ballot(sb0 != kUnset)) + (uint32_t)__popcll(__ballot(sb1 != kUnset));
+            if (ln == 0) atomicAdd(reinterpret_cast<uint32_t*>(&misc[14]), occ);
             __syncthreads();
             const int nso = misc[14];
             const int rs = (int)((2ll * ce * nso + ncand - 1) / max(ncand, 1)) + 4;
             if (2 * rs > nso) return -1;  // uniform: ce is a large share of the ids
             uint64_t prefix;
             int shift;
-            radix_select<kSNT>(kSNT, rs, [&](int i) -> uint64_t { return skey[i]; }, sh, prefix, shift, 32);
+            radix_select<kSNT>(NS, rs, [&](int i) -> uint64_t { return skey[i]; }, sh, prefix, shift, 32);
             const uint32_t lkey = (uint32_t)(prefix >> 32);
             for (int i0 = 0; i0 < T; i0 += kEmitU * kSNT) {  // uniform trip count (wave-level append)
                 uint32_t tm = 0;
