@@ -381,7 +381,7 @@ struct Core {
 
     // Picks the LDS score-tile size and grid for a search with the given bounds:
     // the largest tile that fits the per-workgroup LDS share (wg_per_cu workgroups
-    // of 512 threads per CU; default 1, i.e. a 32K-slot tile of 4-byte ids).
+    // of 512 threads per CU; default 1): the most 4-slot buckets, a multiple of 64.
     ceos::QueryConfig search_config(int s_max, int c_max, int k_max) {
         ceos::QueryConfig cfg = search_config_with(s_max, c_max, k_max, wg_per_cu, table_log2);
         arm_timing(cfg);
