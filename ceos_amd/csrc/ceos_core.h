@@ -149,6 +149,7 @@ struct ScratchSet {
     DBuf<uint32_t> tkeys;
     DBuf<int32_t> probes;
     DBuf<int32_t> work, status;
+    DBuf<float> est;
 
     ceos::QueryScratch ensure(const ceos::QueryConfig& cfg, int nq, int DR) {
         cand.ensure((size_t)cfg.grid * cfg.cand_stride);
@@ -162,7 +163,12 @@ struct ScratchSet {
             status.alloc(1);
             HIP_CHECK(hipMemset(status.p, 0, 4));
         }
-        return ceos::QueryScratch{qproj.p, probes.p, cand.p, pairs.p, stash.p, tkeys.p, work.p, status.p};
+        if (!est.p) {
+            est.alloc(1);
+            const float half = 0.5f;
+            HIP_CHECK(hipMemcpy(est.p, &half, 4, hipMemcpyHostToDevice));
+        }
+        return ceos::QueryScratch{qproj.p, probes.p, cand.p, pairs.p, stash.p, tkeys.p, work.p, status.p, est.p};
     }
     void release() {
         cand.release(); qproj.release(); pairs.release(); stash.release(); tkeys.release(); probes.release();

@@ -118,6 +118,7 @@ struct QueryScratch {
     uint32_t* keys;    // grid * search_tile_slots_nbk(tile_buckets) tile ids during the ordered pass
     int32_t* work;     // work counter (1 int)
     int32_t* status;   // error flags (1 int): bit0 = hash table overflow
+    float* est;        // running distinct-ids-per-entry estimate carried across launches (1 float)
 };
 
 // Fused coCEOs search (projection, probe selection, score accumulation, candidate

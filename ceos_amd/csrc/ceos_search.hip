@@ -285,7 +285,12 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
     uint32_t* keys_g = scr.keys + (int64_t)blockIdx.x * T;
     uint4* stash_list = scr.stash + (int64_t)blockIdx.x * kStashList;
     const int tid = threadIdx.x, wave = tid / kWave, ln = tid % kWave;
-    float frac = 0.5f;  // running estimate of distinct ids / entries for this workgroup
+    // running estimate of distinct ids / entries: starts from the value the last launch on
+    // this scratch left (small batches, as in serving, then size their passes from what
+    // earlier queries saw) and is written back when the workgroup finishes
+    float frac = scr.est ? fminf(1.0f, fmaxf(0.01f, scr.est[0])) : 0.5f;
+    bool did_query = false;
+    int nq_done = 0;  // queries this workgroup finished in this launch
     const int nq_run = qb.nq_dev ? min(qb.nq, qb.nq_dev[0]) : qb.nq;
     // CEOs-Hash: candidates are all distinct ids (no scores); their ids go to idlist
     // (the `pairs` scratch, unused in this mode) and their rerank keys to `cand`
@@ -298,6 +303,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         const int q = misc[0];
         __syncthreads();
         if (q >= nq_run) break;
+        did_query = true;
         // per-query parameters, clamped to the bounds the launch was sized for
         const int kq = min(max(qb.k ? qb.k[q] : qb.k0, 0), cfg.k_max);
         const int sq = min(max(qb.s ? qb.s[q] : qb.s0, 0), cfg.s_max);
@@ -328,8 +334,13 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         __syncthreads();
         const int E = misc[1];
         // hash partitions: the expected distinct ids of a pass fill at most 90% of the tile
-        // (an overflow restarts the pass with twice the partitions)
-        uint32_t P = (uint32_t)max(1, (int)ceilf((float)E * fminf(1.0f, frac) / (0.9f * (float)T)));
+        // (an overflow restarts the pass with twice the partitions). A workgroup's first
+        // query in a launch plans for 25% more ids and 70% load: with one query per
+        // workgroup (small batches) the launch takes as long as its slowest query, and a
+        // restart would double it.
+        const bool first = nq_done == 0;
+        uint32_t P = (uint32_t)max(1, (int)ceilf((float)E * fminf(1.0f, frac * (first ? 1.25f : 1.0f)) /
+                                                 ((first ? 0.7f : 0.9f) * (float)T)));
         const int mp = qb.mp;
         const int jlast = sq - 1;
 
@@ -1236,7 +1247,9 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             qb.out_ip[off + t] = t < kk ? ckey_val(tk[t]) : 0.0f;
         }
         __syncthreads();
+        ++nq_done;
     }
+    if (tid == 0 && did_query && scr.est) scr.est[0] = frac;  // any workgroup's value will do
 }
 
 int search_ept(int mp) {
