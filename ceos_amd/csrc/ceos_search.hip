@@ -1224,6 +1224,17 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         if (kk > 0) {
             // one instantiation per key array (LDS rk, or global cand in hash mode)
             auto topk = [&](auto get) __attribute__((always_inline)) {
+                if (nr <= kSNT) {  // uniform: one key per thread, placed by its rank (keys are unique)
+                    uint64_t k = 0;
+                    int r = 0;
+                    if (tid < nr) {
+                        k = get(tid);
+                        for (int j = 0; j < nr; ++j) r += get(j) > k ? 1 : 0;
+                    }
+                    if (tid < nr && r < kk) tk[r] = k;
+                    __syncthreads();
+                    return;
+                }
                 uint64_t prefix;
                 int shift;
                 radix_select<kSNT>(nr, kk, get, sh, prefix, shift);
