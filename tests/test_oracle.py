@@ -117,7 +117,7 @@ def test_projection_is_sign_fwht_composition(oracle):
         err = np.abs(P[:, r * D:(r + 1) * D] - exp)
         bound = 3 * 6 * 2.0 ** -24 * np.abs(xp).sum(1, keepdims=True) * F * 2
         assert (err <= bound).all()
-    # a planted mistake (sign of round 2 dropped) is caught
+    # a planted mistake (the second sign-flip-and-FWHT step loses its signs) is caught
     s_wrong = bits.copy(); s_wrong[F:2 * F] = 1
     M = np.eye(F)
     for i in range(3):
