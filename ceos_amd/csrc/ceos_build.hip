@@ -57,12 +57,14 @@ __global__ __launch_bounds__(NW * kWave) void k_project_rows(IndexDims dm, const
                                                              int64_t first, int64_t stride, int64_t nrows,
                                                              const uint32_t* __restrict__ signs,
                                                              float* __restrict__ out, int64_t ld_out,
-                                                             int col_major) {
+                                                             int col_major, const int32_t* __restrict__ nrows_dev) {
     using P = Proj<L>;
     extern __shared__ float lds_all[];
     const int wave = threadIdx.x / kWave, ln = threadIdx.x % kWave;
     float* lds = lds_all + wave * P::kLdsWords;
     const int W = sign_words(L);
+    // captured plans size the grid for their largest batch and read the actual row count here
+    if (nrows_dev) nrows = min(nrows, (int64_t)max(0, nrows_dev[0]));
     const int64_t total = nrows * dm.R;
     for (int64_t p = (int64_t)blockIdx.x * NW + wave; p < total; p += (int64_t)gridDim.x * NW) {
         const int64_t k = p / dm.R;
@@ -87,26 +89,29 @@ __global__ __launch_bounds__(NW * kWave) void k_project_rows(IndexDims dm, const
 template <int NW>
 static hipError_t launch_project_rows_nw(const IndexDims& dm, const float* X, int64_t first, int64_t stride,
                                          int64_t nrows, const uint32_t* signs, float* out, int64_t ld_out,
-                                         bool col_major, hipStream_t st) {
+                                         bool col_major, hipStream_t st, const int32_t* nrows_dev) {
     const int64_t waves = nrows * dm.R;
     const int grid = (int)std::min<int64_t>((waves + NW - 1) / NW, 256 * 64 / NW);
     CEOS_SWITCH_L(dm.L, {
         const size_t lds = (size_t)NW * Proj<LL>::kLdsWords * sizeof(float);
         allow_lds(k_project_rows<LL, NW>, lds);
         hipLaunchKernelGGL((k_project_rows<LL, NW>), dim3(grid), dim3(NW * kWave), lds, st, dm, X, first, stride,
-                           nrows, signs, out, ld_out, (int)col_major);
+                           nrows, signs, out, ld_out, (int)col_major, nrows_dev);
     });
     return hipGetLastError();
 }
 
 hipError_t launch_project_rows(const IndexDims& dm, const float* X, int64_t first, int64_t stride,
                                int64_t nrows, const uint32_t* signs, float* out, int64_t ld_out,
-                               bool col_major, hipStream_t st, int waves_per_wg) {
+                               bool col_major, hipStream_t st, int waves_per_wg, const int32_t* nrows_dev) {
     if (nrows <= 0) return hipSuccess;
     switch (waves_per_wg) {
-        case 2: return launch_project_rows_nw<2>(dm, X, first, stride, nrows, signs, out, ld_out, col_major, st);
-        case 8: return launch_project_rows_nw<8>(dm, X, first, stride, nrows, signs, out, ld_out, col_major, st);
-        default: return launch_project_rows_nw<4>(dm, X, first, stride, nrows, signs, out, ld_out, col_major, st);
+        case 2:
+            return launch_project_rows_nw<2>(dm, X, first, stride, nrows, signs, out, ld_out, col_major, st, nrows_dev);
+        case 8:
+            return launch_project_rows_nw<8>(dm, X, first, stride, nrows, signs, out, ld_out, col_major, st, nrows_dev);
+        default:
+            return launch_project_rows_nw<4>(dm, X, first, stride, nrows, signs, out, ld_out, col_major, st, nrows_dev);
     }
 }
 

@@ -151,6 +151,13 @@ struct ScratchSet {
     DBuf<int32_t> work, status;
     DBuf<float> est;
 
+    bool fits(const ceos::QueryConfig& cfg, int nq, int DR) const {
+        return cand.n >= (size_t)cfg.grid * cfg.cand_stride && pairs.n >= (size_t)cfg.grid * cfg.cand_stride &&
+               stash.n >= (size_t)cfg.grid * ceos::kSearchStashList &&
+               tkeys.n >= (size_t)cfg.grid * ceos::search_tile_slots_nbk(cfg.tile_buckets) &&
+               qproj.n >= (size_t)std::max(nq, 1) * DR && probes.n >= (size_t)std::max(nq, 1) * cfg.s_max && work.p &&
+               status.p && est.p;
+    }
     ceos::QueryScratch ensure(const ceos::QueryConfig& cfg, int nq, int DR) {
         cand.ensure((size_t)cfg.grid * cfg.cand_stride);
         pairs.ensure((size_t)cfg.grid * cfg.cand_stride);
@@ -225,8 +232,69 @@ struct Core {
 
     std::mutex mu;
 
+    // Cross-stream ordering. Device searches may run on caller streams and return before
+    // they finish; they share this index's query scratch, and updates rewrite the lists
+    // and rows every search reads. Each launch that reads the index records an event on
+    // its stream. Work that rewrites or frees the index (builds, updates, deletes, loads,
+    // clear, destruction) first waits on the host for every such event, and a launch
+    // that uses the index's own scratch first makes its stream wait for the previous
+    // scratch user when that ran on another stream.
+    struct StreamEvent {
+        hipStream_t s;
+        hipEvent_t e;
+        bool pending;
+    };
+    std::vector<StreamEvent> rd_ev;
+    hipEvent_t scr_ev = nullptr;
+    hipStream_t scr_stream = nullptr;
+    bool scr_pending = false;
+
+    void note_read(hipStream_t s_) {
+        StreamEvent* se = nullptr;
+        for (auto& x : rd_ev)
+            if (x.s == s_) se = &x;
+        if (!se) {
+            if (rd_ev.size() >= 64) wait_reads();  // many caller streams: retire the old events
+            if (rd_ev.size() >= 64) {
+                for (auto& x : rd_ev) (void)hipEventDestroy(x.e);
+                rd_ev.clear();
+            }
+            hipEvent_t e = nullptr;
+            HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            rd_ev.push_back({s_, e, false});
+            se = &rd_ev.back();
+        }
+        HIP_CHECK(hipEventRecord(se->e, s_));
+        se->pending = true;
+    }
+    // Host wait for every launch that reads the index (before it is rewritten or freed).
+    void wait_reads() {
+        for (auto& x : rd_ev)
+            if (x.pending) {
+                HIP_CHECK(hipEventSynchronize(x.e));
+                x.pending = false;
+            }
+        scr_pending = false;
+    }
+    // Orders a launch that uses the index's scratch after the previous one.
+    void before_scratch(hipStream_t s_) {
+        if (scr_pending && scr_stream != s_) HIP_CHECK(hipStreamWaitEvent(s_, scr_ev, 0));
+    }
+    void after_scratch(hipStream_t s_) {
+        if (!scr_ev) HIP_CHECK(hipEventCreateWithFlags(&scr_ev, hipEventDisableTiming));
+        HIP_CHECK(hipEventRecord(scr_ev, s_));
+        scr_stream = s_;
+        scr_pending = true;
+        note_read(s_);
+    }
+
     Core() = default;
     ~Core() {
+        for (auto& x : rd_ev) {
+            if (x.pending) (void)hipEventSynchronize(x.e);
+            (void)hipEventDestroy(x.e);
+        }
+        if (scr_ev) (void)hipEventDestroy(scr_ev);
         for (auto& e : kev)
             for (hipEvent_t x : e) (void)hipEventDestroy(x);
         if (st_) (void)hipStreamDestroy(st_);
@@ -271,6 +339,7 @@ struct Core {
     }
 
     void clear() {
+        wait_reads();
         bkt.release(); bsize.release(); X.release(); signs.release(); dead.release();
         Xq.release(); Xqs.release();
         scr.release(); estP.release();
@@ -431,7 +500,12 @@ struct Core {
         return cfg;
     }
 
-    ceos::QueryScratch scratch(const ceos::QueryConfig& cfg, int nq) { return scr.ensure(cfg, nq, DR()); }
+    // The index's scratch sized for a launch; a launch still in flight keeps its buffers
+    // until it has finished.
+    ceos::QueryScratch scratch(const ceos::QueryConfig& cfg, int nq) {
+        if (scr_pending && !scr.fits(cfg, nq, DR())) HIP_CHECK(hipEventSynchronize(scr_ev));
+        return scr.ensure(cfg, nq, DR());
+    }
 
     ceos::RowRing rows() const { return ceos::RowRing{X.p, cap, head, base, live, Xq.p, Xqs.p}; }
 
@@ -453,6 +527,7 @@ struct Core {
         HIP_CHECK(ceos::launch_quantize_rows(X.p, slot0, count, cap, d, Xq.p, Xqs.p, s_));
     }
     void set_rerank_bounds(bool on) {
+        wait_reads();
         rerank_bounds = on;
         if (!on) {
             Xq.release();
@@ -472,8 +547,15 @@ struct Core {
         }
     }
 
-    void validate_query(int s_, int mp_, int c_, int k, int s_limit, const char* s_limit_name) const {
+    // The coCEOs / CEOs-Hash lists exist (a CEOs-Est build keeps dense projections instead).
+    void require_lists() const {
         if (!built) throw StateError("index is not built");
+        if (!bkt.p || !bsize.p || !signs.p || !X.p)
+            throw StateError("coCEOs / CEOs-Hash index is not built (build_coCEOs_Est or build_CEOs_Hash)");
+    }
+
+    void validate_query(int s_, int mp_, int c_, int k, int s_limit, const char* s_limit_name) const {
+        require_lists();
         if (mp_ > m) throw ArgError("Number of probed points must be smaller than number of indexed top-m points");
         if (s_ > s_limit)
             throw ArgError(std::string("Number of probed vectors must be smaller than ") + s_limit_name);
@@ -490,9 +572,11 @@ struct Core {
         if (nq <= 0 || k == 0) return;
         if (hash) c_ = 1;
         const ceos::QueryConfig cfg = search_config(s_, c_, k);
+        before_scratch(s__);
         const ceos::QueryScratch scr = scratch(cfg, nq);
         ceos::QueryBatch qb{dQ, nq, nullptr, nullptr, nullptr, k, s_, c_, mp, nullptr, ids, ips, nullptr, hash ? 1 : 0};
         HIP_CHECK(ceos::launch_coceos_search(dims(), signs.p, bkt.p, bsize.p, rows(), qb, cfg, scr, s__));
+        after_scratch(s__);
     }
 
     void search_host(const float* Q, int nq, int k, int s_, int c_, int32_t* ids, float* ips, bool hash = false) {
@@ -566,42 +650,76 @@ struct Core {
     int64_t load(const char* path, int kind) {
         FILE* f = std::fopen(path, "rb");
         if (!f) throw ArgError(std::string("cannot open ") + path);
+        struct Closer {
+            FILE* f;
+            ~Closer() { std::fclose(f); }
+        } closer{f};
         auto get = [&](void* p, size_t bytes) {
-            if (bytes && std::fread(p, 1, bytes, f) != bytes) {
-                std::fclose(f);
+            if (bytes && std::fread(p, 1, bytes, f) != bytes)
                 throw ArgError(std::string(path) + ": truncated index file");
-            }
         };
+        auto invalid = [&](const char* what) { throw ArgError(std::string(path) + ": invalid " + what + " in the index file"); };
         char magic[8];
         int32_t hdr[11];
         uint32_t seed_v;
         int64_t n;
         get(magic, 8);
-        if (std::memcmp(magic, kMagic, 8) != 0) {
-            std::fclose(f);
-            throw ArgError(std::string(path) + " is not an index file");
-        }
+        if (std::memcmp(magic, kMagic, 8) != 0) throw ArgError(std::string(path) + " is not an index file");
         get(hdr, sizeof(hdr));
         get(&seed_v, 4);
         get(&n, 8);
-        if (hdr[0] != kind) {
-            std::fclose(f);
+        if (hdr[0] != kind)
             throw ArgError(std::string(path) + (kind ? " holds a CEOs index, not a streamCEOs one"
                                                      : " holds a streamCEOs index, not a CEOs one"));
-        }
-        if (hdr[1] != d) {
-            std::fclose(f);
+        if (hdr[1] != d)
             throw ArgError("the index file has " + std::to_string(hdr[1]) + " features, the handle " +
                            std::to_string(d));
+        // Everything in the file is checked before anything is allocated or copied to the
+        // device: a foreign or damaged file must not size allocations or make the search
+        // index rows out of bounds. The rows are skipped first and read last.
+        const int fD = hdr[2], fR = hdr[3], fm = hdr[4], fF = hdr[5], fL = hdr[6];
+        if (n < 0 || n >= (int64_t(1) << 31)) invalid("point count");
+        if (fD <= 0 || fR <= 0 || fm <= 0) invalid("n_proj / n_repeats / top_m");
+        if (fm > kMaxTopM || fm > n) invalid("top_m");
+        if ((int64_t)fD * fR > (int64_t(1) << 20)) invalid("n_proj * n_repeats");
+        if (fht_dim(fD, d) > 4096 || fF != fht_dim(fD, d) || fL < 0 || fL > 12 || (1 << fL) != fF) invalid("fhtDim");
+        if (hdr[7] <= 0 || hdr[8] <= 0 || hdr[9] <= 0) invalid("query parameters");
+        const int64_t rows_at = (int64_t)ftello(f);
+        if (rows_at < 0 || fseeko(f, (off_t)(rows_at + n * (int64_t)d * 4), SEEK_SET) != 0)
+            throw ArgError(std::string(path) + ": truncated index file");
+        const int nl = 2 * fD * fR;
+        std::vector<Entry> h((size_t)nl * fm);
+        std::vector<int32_t> sz(nl);
+        std::vector<uint32_t> flags((size_t)(n + 31) / 32);
+        get(h.data(), h.size() * sizeof(Entry));
+        get(sz.data(), sz.size() * 4);
+        get(flags.data(), flags.size() * 4);
+        std::vector<float> ctr;
+        if (kind == 1 && hdr[10] != 0) {
+            ctr.resize(d);
+            get(ctr.data(), (size_t)d * 4);
         }
+        // every list holds at most top_m valid entries, each a stored point
+        for (int l = 0; l < nl; ++l) {
+            if (sz[l] < 0 || sz[l] > fm) invalid("bucket size");
+            for (int t = 0; t < sz[l]; ++t)
+                if ((int64_t)h[(size_t)l * fm + t].x >= n) invalid("bucket id");
+        }
+        if (fseeko(f, (off_t)rows_at, SEEK_SET) != 0) throw ArgError(std::string(path) + ": cannot read the rows");
+
+        wait_reads();
         built = false;
-        D = hdr[2]; R = hdr[3]; m = hdr[4]; F = hdr[5]; L = hdr[6];
-        s = hdr[7]; mp = hdr[8]; c = hdr[9]; centering = hdr[10] != 0;
-        seed = (int)(seed_v & 0x7fffffffu) == (int)seed_v ? (int)seed_v : -1;
-        params_set = true;
+        bkt.release();
+        bsize.release();
+        dead.release();
         estP.release();
         Xq.release();
         Xqs.release();
+        D = fD; R = fR; m = fm; F = fF; L = fL;
+        s = hdr[7]; mp = hdr[8]; c = hdr[9]; centering = hdr[10] != 0;
+        seed = (int)(seed_v & 0x7fffffffu) == (int)seed_v ? (int)seed_v : -1;
+        params_set = true;
+        center = std::move(ctr);
         X.alloc((size_t)std::max<int64_t>(n, 1) * d);
         cap = std::max<int64_t>(n, 1); head = 0; base = 0; live = (uint32_t)n;
         const int64_t chunk = std::max<int64_t>(1, (int64_t(64) << 20) / (4LL * d));
@@ -612,24 +730,10 @@ struct Core {
             get(buf.data(), buf.size() * 4);
             HIP_CHECK(hipMemcpy(X.p + i * d, buf.data(), buf.size() * 4, hipMemcpyHostToDevice));
         }
-        const int nl = 2 * DR();
-        std::vector<Entry> h((size_t)nl * m);
-        std::vector<int32_t> sz(nl);
-        std::vector<uint32_t> flags((size_t)(n + 31) / 32);
-        get(h.data(), h.size() * sizeof(Entry));
-        get(sz.data(), sz.size() * 4);
-        get(flags.data(), flags.size() * 4);
-        center.clear();
-        if (kind == 1 && centering) {
-            center.resize(d);
-            get(center.data(), (size_t)d * 4);
-        }
-        std::fclose(f);
         bkt.alloc(h.size());
         bsize.alloc(sz.size());
         HIP_CHECK(hipMemcpy(bkt.p, h.data(), h.size() * sizeof(Entry), hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(bsize.p, sz.data(), sz.size() * 4, hipMemcpyHostToDevice));
-        dead.release();
         if (kind == 1) {  // streams always carry the flags (ring slots = logical rows here)
             flags.resize((size_t)(cap + 31) / 32, 0u);
             dead.alloc(flags.size());
@@ -643,7 +747,7 @@ struct Core {
     }
 
     void get_buckets(int32_t* cid, float* cval, int32_t* fid, float* fval, int32_t* csz, int32_t* fsz) {
-        if (!built) throw StateError("index is not built");
+        require_lists();
         const int DR_ = DR();
         std::vector<Entry> h((size_t)2 * DR_ * m);
         std::vector<int32_t> sz(2 * DR_);
@@ -717,9 +821,23 @@ struct ceos_plan {
     double tuned_ms = 0.0;
     int wg = 0, tl = 0, pw = 4;  // launch shape: workgroups per CU, tile log2, projection waves per workgroup
 
+    // the last run: later runs on another stream wait for it; capture and destruction
+    // wait for it on the host (the plan's scratch and graph are in use until it ends)
+    hipEvent_t run_ev = nullptr;
+    hipStream_t run_stream = nullptr;
+    bool run_pending = false;
+
     ~ceos_plan() {
+        if (run_ev) {
+            if (run_pending) (void)hipEventSynchronize(run_ev);
+            (void)hipEventDestroy(run_ev);
+        }
         drop();
         if (cap) (void)hipStreamDestroy(cap);
+    }
+    void wait_runs() {
+        if (run_pending) HIP_CHECK(hipEventSynchronize(run_ev));
+        run_pending = false;
     }
     void drop() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -733,6 +851,7 @@ struct ceos_plan {
                     x.L, x.d};
     }
     void capture(int wg_, int tl_, int pw_) {
+        wait_runs();
         drop();
         Core& x = *core;
         cfg = x.search_config_with(s_max, c_max, k_max, wg_, tl_);
@@ -759,8 +878,14 @@ struct ceos_plan {
         pw = pw_;
     }
     void launch(int nq, hipStream_t st) {
+        if (run_pending && run_stream != st) HIP_CHECK(hipStreamWaitEvent(st, run_ev, 0));
         HIP_CHECK(ceos::launch_fill_i32(nq_dev.p, nq, 1, st));  // a kernel, outside the graph
         HIP_CHECK(hipGraphLaunch(exec, st));
+        if (!run_ev) HIP_CHECK(hipEventCreateWithFlags(&run_ev, hipEventDisableTiming));
+        HIP_CHECK(hipEventRecord(run_ev, st));
+        run_stream = st;
+        run_pending = true;
+        core->note_read(st);
     }
     // Times each candidate launch shape once on this batch and keeps the fastest. First
     // the fused kernel's shape (workgroups per CU, score-tile size), then, with that, the

@@ -38,10 +38,12 @@ struct RowRing {
 // Projects rows row(k) = first + k*stride (k < nrows) for all R repeats.
 // col_major: out[col * ld_out + k]; else out[k * ld_out + col], col = r*D + j.
 // waves_per_wg: rows projected together per workgroup (2, 4 or 8; the query projection's
-// tuned launch shape).
+// tuned launch shape). nrows_dev (optional): a row count read on the device, at most
+// nrows (captured plans launch for their largest batch).
 hipError_t launch_project_rows(const IndexDims& dm, const float* X, int64_t first, int64_t stride,
                                int64_t nrows, const uint32_t* signs, float* out, int64_t ld_out,
-                               bool col_major, hipStream_t st, int waves_per_wg = 4);
+                               bool col_major, hipStream_t st, int waves_per_wg = 4,
+                               const int32_t* nrows_dev = nullptr);
 
 // out[c * ld_out + r] = in[r * cols + c] (rows <= 65535 * 64; ld_out >= rows, 0 = rows).
 hipError_t launch_transpose(const float* in, int64_t rows, int64_t cols, float* out, hipStream_t st,

@@ -84,6 +84,7 @@ static void build_coceos_impl(ceos_index* h, const float* X, int64_t n, int d, b
     check_matrix(X, n, d, c.d);
     if (n <= 0) throw ArgError("empty dataset");
     if (n >= (int64_t(1) << 31)) throw ArgError("more than 2^31-1 points");
+    c.wait_reads();
     c.built = false;
     h->n_points = (int)n;
     c.estP.release();
@@ -201,10 +202,12 @@ int ceos_search_batched_device(ceos_index* h, const float* dQ, int nq, int d, co
         if (nq <= 0) return;
         if (!d_k || !d_s || !d_c || !d_off) throw ArgError("per-query arrays must not be null");
         const ceos::QueryConfig cfg = c.search_config(s_max, c_max, std::max(1, k_max));
+        hipStream_t st = c.stream(stream);
+        c.before_scratch(st);
         const ceos::QueryScratch scr = c.scratch(cfg, nq);
         ceos::QueryBatch qb{dQ, nq, d_k, d_s, d_c, 0, 0, 0, c.mp, d_off, d_ids, d_ips};
-        HIP_CHECK(ceos::launch_coceos_search(c.dims(), c.signs.p, c.bkt.p, c.bsize.p, c.rows(), qb, cfg, scr,
-                                             c.stream(stream)));
+        HIP_CHECK(ceos::launch_coceos_search(c.dims(), c.signs.p, c.bkt.p, c.bsize.p, c.rows(), qb, cfg, scr, st));
+        c.after_scratch(st);
     });
 }
 
@@ -240,9 +243,11 @@ int ceos_search_batched(ceos_index* h, const float* Q, int nq, int d, const int3
         HIP_CHECK(hipMemcpyAsync(dc.p, c_, nq * 4, hipMemcpyHostToDevice, c.st()));
         HIP_CHECK(hipMemcpyAsync(doff.p, off.data(), nq * 8, hipMemcpyHostToDevice, c.st()));
         const ceos::QueryConfig cfg = c.search_config(s_max, c_max, std::max(1, k_max));
+        c.before_scratch(c.st());
         const ceos::QueryScratch scr = c.scratch(cfg, nq);
         ceos::QueryBatch qb{dQ.p, nq, dk.p, ds.p, dc.p, 0, 0, 0, c.mp, doff.p, dI.p, dP.p};
         HIP_CHECK(ceos::launch_coceos_search(c.dims(), c.signs.p, c.bkt.p, c.bsize.p, c.rows(), qb, cfg, scr, c.st()));
+        c.after_scratch(c.st());
         HIP_CHECK(hipMemcpyAsync(ids, dI.p, tot * 4, hipMemcpyDeviceToHost, c.st()));
         HIP_CHECK(hipMemcpyAsync(ips, dP.p, tot * 4, hipMemcpyDeviceToHost, c.st()));
         HIP_CHECK(hipStreamSynchronize(c.st()));
@@ -258,6 +263,7 @@ int ceos_build_est(ceos_index* h, const float* X, int64_t n, int d) {
         if (!c.params_set) throw StateError("setIndexParam must be called before build");
         check_matrix(X, n, d, c.d);
         if (n <= 0) throw ArgError("empty dataset");
+        c.wait_reads();
         c.built = false;
         h->n_points = (int)n;
         c.bkt.release();
@@ -294,6 +300,7 @@ int ceos_search_est(ceos_index* h, const float* Q, int nq, int d, int k, int32_t
         if (c.s <= 0 || c.c <= 0) throw ArgError("n_probed_vectors and n_cand must be positive");
         if (k < 0 || k > kMaxK) throw ArgError("n_neighbors out of range");
         if (nq <= 0 || k == 0) return;
+        c.wait_reads();  // the scratch below is shared with device searches
         const int cc = (int)std::min<int64_t>(c.c, c.est_n);
         const int lds = ceos::est_search_lds_bytes(c.dims(), c.s, cc, k);
         if (lds > kMaxLdsBytes) throw ArgError("n_cand too large for the device path");
@@ -541,6 +548,7 @@ int ceos_stream_build(ceos_stream* h, const float* X, int64_t n, int d) {
         check_matrix(X, n, d, c.d);
         if (n <= 0) throw ArgError("empty dataset");
         if (n >= (int64_t(1) << 31)) throw ArgError("more than 2^31-1 points");
+        c.wait_reads();
         c.built = false;
         const int64_t capacity = std::max<int64_t>(n + n / 4, 1024);
         // centering: rows are stored minus the build rows' column mean, which stays
@@ -588,6 +596,7 @@ int ceos_stream_update(ceos_stream* h, const float* Xn, int64_t n_new, int d, in
             Xn = Xc.data();
         }
         if (n_cur + n_new >= (int64_t(1) << 31)) throw ArgError("more than 2^31-1 live points");
+        c.wait_reads();
         hipStream_t st = c.st();
         // grow the ring when the new rows do not fit
         if (n_cur + n_new > c.cap) {
@@ -669,6 +678,7 @@ int ceos_stream_delete_ids(ceos_stream* h, const int32_t* ids, int64_t count) {
         if (!c.built) throw StateError("stream index is not built");
         if (count < 0 || (!ids && count > 0)) throw ArgError("invalid id list");
         if (count == 0) return;
+        c.wait_reads();
         std::vector<uint32_t> dh(c.dead.n);
         HIP_CHECK(hipMemcpy(dh.data(), c.dead.p, dh.size() * 4, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < count; ++i) {

@@ -1278,7 +1278,7 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
     if (mp > kSNT * kBufEntries) return hipErrorInvalidValue;
     hipError_t err = hipSuccess;
     if (cfg.ev[0] && (err = hipEventRecord(cfg.ev[0], st)) != hipSuccess) return err;
-    err = launch_project_rows(dm, qb.Q, 0, 1, qb.nq, signs, scr.qproj, DR, false, st, cfg.proj_waves);
+    err = launch_project_rows(dm, qb.Q, 0, 1, qb.nq, signs, scr.qproj, DR, false, st, cfg.proj_waves, qb.nq_dev);
     if (err != hipSuccess) return err;
     {
         const int pgrid = std::min(qb.nq, 4096);

@@ -178,7 +178,9 @@ struct ceos_server {
         for (size_t i = 0; i < sl.tickets.size(); ++i) {
             int idx = 0;
             Feed* f = find(sl.tickets[i], &idx);
-            if (!f) continue;
+            // a request failed (fail_all) or already collected while its batch ran keeps
+            // that state: it is neither given results nor counted again
+            if (!f || f->state[idx] != Feed::kQueued) continue;
             const int kq = f->k[idx];
             f->ids[idx].assign(sl.hI.p + i * cfg.k_max, sl.hI.p + i * cfg.k_max + kq);
             f->ips[idx].assign(sl.hP.p + i * cfg.k_max, sl.hP.p + i * cfg.k_max + kq);
@@ -348,7 +350,10 @@ int make_server(Core& x, ceos_index* hi, ceos_stream* hs, const ceos_server_conf
         if (!out || !cfg) throw ArgError("null argument");
         if (cfg->max_batch <= 0 || cfg->k_max <= 0 || cfg->s_max <= 0 || cfg->c_max <= 0)
             throw ArgError("max_batch, k_max, s_max and c_max must be positive");
-        if (!x.built) throw StateError("index is not built");
+        {
+            std::lock_guard<std::mutex> g(x.mu);
+            x.require_lists();
+        }
         auto srv = std::make_unique<ceos_server>();
         srv->core = &x;
         srv->index = hi;

@@ -107,3 +107,52 @@ def test_rerank_bounds_toggle_without_device(CEOs):
     assert s.rerank_bounds is True
     s.rerank_bounds = False
     assert s.rerank_bounds is False
+
+
+def _index_file(path, n=20, d=8, D=64, R=1, m=10, F=64, L=6, ids=None, sizes=None, truncate=0):
+    """Bytes of a coCEOs index file (the layout Core::save writes)."""
+    rng = np.random.default_rng(5)
+    nl = 2 * D * R
+    ent = np.zeros((nl, m, 2), np.uint32)
+    ent[:, :, 0] = rng.integers(0, max(n, 1), (nl, m)) if ids is None else ids
+    ent[:, :, 1] = rng.standard_normal((nl, m)).astype(np.float32).view(np.uint32)
+    sz = np.full(nl, m, np.int32) if sizes is None else np.asarray(sizes, np.int32)
+    blob = b"CEOSIDX1" + np.array([0, d, D, R, m, F, L, 10, 10, 10, 0], np.int32).tobytes()
+    blob += np.uint32(42).tobytes() + np.int64(n).tobytes()
+    blob += rng.standard_normal((max(n, 0), d)).astype(np.float32).tobytes()
+    blob += ent.tobytes() + sz.tobytes() + np.zeros((max(n, 0) + 31) // 32, np.uint32).tobytes()
+    with open(path, "wb") as f:
+        f.write(blob[: len(blob) - truncate])
+
+
+@pytest.mark.parametrize("case", ["id_out_of_range", "size_above_top_m", "top_m_above_n", "wrong_fht_dim",
+                                  "truncated", "negative_n"])
+def test_load_rejects_damaged_files_before_device_work(tmp_path, case):
+    """Every field of an index file is checked before anything reaches the device, so a
+    damaged or foreign file fails with CEOS_EINVAL on a host without a GPU."""
+    path = str(tmp_path / "bad.ceos")
+    if case == "id_out_of_range":
+        ids = np.zeros((128, 10), np.uint32)
+        ids[77, 3] = 20  # n = 20
+        _index_file(path, ids=ids)
+    elif case == "size_above_top_m":
+        sizes = np.full(128, 10)
+        sizes[5] = 11
+        _index_file(path, sizes=sizes)
+    elif case == "top_m_above_n":
+        _index_file(path, n=8, m=10)
+    elif case == "wrong_fht_dim":
+        _index_file(path, F=128, L=7)
+    elif case == "truncated":
+        _index_file(path, truncate=9)
+    else:
+        _index_file(path, n=-1)
+    L = capi.lib()
+    L.ceos_load.argtypes = [C.c_void_p, C.c_char_p]
+    h = C.c_void_p()
+    assert L.ceos_create(1, 8, C.byref(h)) == 0
+    try:
+        assert L.ceos_load(h, path.encode()) == 1, L.ceos_last_error()
+        assert b"index file" in L.ceos_last_error()
+    finally:
+        L.ceos_destroy(h)
