@@ -20,7 +20,8 @@ PYMOD = os.path.join(HERE, "CEOs" + EXT_SUFFIX)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["ceos_build.hip", "ceos_query.hip", "ceos_search.hip", "ceos_serve.cpp", "ceos_runtime.cpp"]
+HIP_SOURCES = ["ceos_build.hip", "ceos_query.hip", "ceos_search.hip", "ceos_static.hip", "ceos_serve.cpp",
+               "ceos_runtime.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -39,6 +39,7 @@ def lib():
             "ceos_build_coceos_device": (i32, [vp, P, i64, i32, vp]),
             "ceos_search_coceos": (i32, [vp, P, i32, i32, i32, P, P]),
             "ceos_search_coceos_device": (i32, [vp, P, i32, i32, i32, P, P, vp]),
+            "ceos_search_coceos_device_noalloc": (i32, [vp, P, i32, i32, i32, P, P, vp]),
             "ceos_search_batched": (i32, [vp, P, i32, i32, P, P, P, P, P]),
             "ceos_search_batched_device": (i32, [vp, P, i32, i32, P, P, P, P, i32, i32, i32, P, P, vp]),
             "ceos_set_search_config": (i32, [vp, i32, i32]),

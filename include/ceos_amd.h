@@ -74,6 +74,20 @@ int ceos_search_coceos(ceos_index* h, const float* Q, int nq, int d, int k, int3
 int ceos_search_coceos_device(ceos_index* h, const float* dQ, int nq, int d, int k, int32_t* d_ids,
                               float* d_ips, void* stream);
 
+/* Allocation-free form of ceos_search_coceos_device. No reference counterpart: it is for
+ * callers that cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments and results; it calls no
+ * hipMalloc, hipFree or allocating library routine. Its search scratch is a static
+ * device array of 64 MiB per GPU, held until the process ends and shared by every call
+ * on that GPU: calls that overlap on one GPU race (calls ordered on one stream do not).
+ * The persistent grid narrows and the queries go in chunks to fit it; a search whose
+ * single workgroup share (about 16 B x n_probed_vectors x n_probed_points + 130 KB)
+ * does not fit fails with CEOS_ENOMEM before any launch. Unlike the allocating form it
+ * records no event: synchronise the stream before updating, rebuilding or freeing the
+ * index. */
+int ceos_search_coceos_device_noalloc(ceos_index* h, const float* dQ, int nq, int d, int k, int32_t* d_ids,
+                                      float* d_ips, void* stream);
+
 /* build_CEOs_Hash(dataset) — python_wrapper.cpp:46, CEOs.cpp:709-902: the coCEOs lists
  * (ids are what CEOs-Hash uses). With centering set, the stored points are X minus its
  * column mean. search_CEOs_Hash(queries, n_neighbors) — python_wrapper.cpp:47-48,
