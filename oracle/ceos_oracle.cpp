@@ -357,6 +357,66 @@ void oracle_ceos_get_bits(void* hv, uint8_t* out) {
     std::memcpy(out, h->P.bits.data(), h->P.bits.size());
 }
 
+// Valid entries per list after load_index (a stream index's lists may be shorter than m).
+void oracle_ceos_set_sizes(void* hv, const int32_t* csize, const int32_t* fsize) {
+    auto* h = (OracleCEOs*)hv;
+    std::memcpy(h->B.csize.data(), csize, (size_t)h->B.DR * 4);
+    std::memcpy(h->B.fsize.data(), fsize, (size_t)h->B.DR * 4);
+}
+
+// src/CEOs.cpp:343-432 restricted to the listed columns: the per-column bounded heaps of
+// the build (close: value, far: -value; replacement on strict '>'), fed with the rows in
+// index order. Rows are projected chunk by chunk and only for the repeats the columns
+// fall in, so single lists of an index too large for the n x DR projection matrix can be
+// checked. Outputs [ncols][m], drained largest first (src/CEOs.cpp:417-428).
+void oracle_ceos_bucket_columns(void* hv, const float* X, int64_t n, const int* cols, int ncols, int32_t* cid,
+                                float* cval, int32_t* fid, float* fval) {
+    auto* h = (OracleCEOs*)hv;
+    const Params& P = h->P;
+    const int L = (int)std::log2((double)P.F);
+    std::vector<int> reps;
+    for (int i = 0; i < ncols; ++i) {
+        const int r = cols[i] / P.D;
+        if (std::find(reps.begin(), reps.end(), r) == reps.end()) reps.push_back(r);
+    }
+    std::vector<MinHeap> close(ncols), far(ncols);
+    const int64_t chunk = 1 << 15;
+    std::vector<float> proj((size_t)chunk * reps.size() * P.D);
+    for (int64_t a = 0; a < n; a += chunk) {
+        const int64_t b = std::min(n, a + chunk);
+#pragma omp parallel for num_threads(P.threads) schedule(static)
+        for (int64_t i = a; i < b; ++i) {
+            std::vector<float> y(P.F);
+            for (size_t ri = 0; ri < reps.size(); ++ri) {
+                std::fill(y.begin(), y.end(), 0.0f);
+                std::memcpy(y.data(), X + (size_t)i * P.d, sizeof(float) * P.d);
+                const size_t b0 = (size_t)P.F * 3 * reps[ri];
+                for (int rd = 0; rd < 3; ++rd) {
+                    for (int j = 0; j < P.F; ++j) y[j] *= (2 * (float)P.bits[b0 + (size_t)rd * P.F + j] - 1);
+                    fwht(y.data(), L);
+                }
+                std::memcpy(&proj[((size_t)(i - a) * reps.size() + ri) * P.D], y.data(), sizeof(float) * P.D);
+            }
+        }
+#pragma omp parallel for num_threads(P.threads) schedule(dynamic, 1)
+        for (int ci = 0; ci < ncols; ++ci) {
+            const int r = cols[ci] / P.D, j = cols[ci] % P.D;
+            const size_t ri = std::find(reps.begin(), reps.end(), r) - reps.begin();
+            for (int64_t i = a; i < b; ++i) {
+                const float v = proj[((size_t)(i - a) * reps.size() + ri) * P.D + j];
+                if ((int)close[ci].size() < P.m) close[ci].emplace((int)i, v);
+                else if (v > close[ci].top().v) { close[ci].pop(); close[ci].emplace((int)i, v); }
+                if ((int)far[ci].size() < P.m) far[ci].emplace((int)i, -v);
+                else if (-v > far[ci].top().v) { far[ci].pop(); far[ci].emplace((int)i, -v); }
+            }
+        }
+    }
+    for (int ci = 0; ci < ncols; ++ci) {
+        drain(close[ci], cid + (size_t)ci * P.m, cval + (size_t)ci * P.m, 0);
+        drain(far[ci], fid + (size_t)ci * P.m, fval + (size_t)ci * P.m, 0);
+    }
+}
+
 // Projections of arbitrary rows with the index's sign diagonals (n x DR row-major).
 void oracle_ceos_project(void* hv, const float* rows, int64_t n, float* out) {
     auto* h = (OracleCEOs*)hv;

@@ -53,6 +53,8 @@ def lib():
             "oracle_ceos_get_bits": (None, [vp, _u8p]),
             "oracle_ceos_load_index": (None, [vp, _f32p, _i32p, _f32p, _i32p, _f32p]),
             "oracle_ceos_project": (None, [vp, _f32p, i64, _f32p]),
+            "oracle_ceos_set_sizes": (None, [vp, _i32p, _i32p]),
+            "oracle_ceos_bucket_columns": (None, [vp, _f32p, i64, _i32p, i32, _i32p, _f32p, _i32p, _f32p]),
             "oracle_ceos_search_coceos": (i32, [vp, _f32p, i32, _i32p, _i32p, i32, _i32p, _i64p,
                                                  _i32p, _f32p, _i32p, _f32p]),
             "oracle_ceos_build_est": (None, [vp, _f32p]),
@@ -159,6 +161,22 @@ class OracleCEOs:
         c = lambda a, t: np.ascontiguousarray(a, dtype=t)
         lib().oracle_ceos_load_index(self._h, _f32(X), c(cid, np.int32), c(cval, np.float32),
                                      c(fid, np.int32), c(fval, np.float32))
+
+    def set_sizes(self, csize, fsize):
+        """Valid entries per close / far list after load_index (default: top_m each)."""
+        lib().oracle_ceos_set_sizes(self._h, np.ascontiguousarray(csize, np.int32),
+                                    np.ascontiguousarray(fsize, np.int32))
+
+    def bucket_columns(self, X, cols):
+        """The build's close / far lists of the given columns over the rows X, projected in
+        chunks (src/CEOs.cpp:343-432): (cid, cval, fid, fval), each [len(cols)][top_m]."""
+        X = _f32(X)
+        cols = np.ascontiguousarray(cols, np.int32)
+        nc, m = len(cols), self.top_m
+        cid = np.zeros((nc, m), np.int32); fid = np.zeros((nc, m), np.int32)
+        cv = np.zeros((nc, m), np.float32); fv = np.zeros((nc, m), np.float32)
+        lib().oracle_ceos_bucket_columns(self._h, X, X.shape[0], cols, nc, cid, cv, fid, fv)
+        return cid, cv, fid, fv
 
     def sign_bits(self):
         out = np.zeros(self.F * 3 * self.R, np.uint8)
