@@ -315,3 +315,18 @@ def test_stream_reference_errors(oracle):
         st.update(np.ones((1, 8), np.float32), 25)      # n_del > size
     with pytest.raises(ValueError):
         st.update(np.ones((1, 8), np.float32), 15)      # fewer than top_m left
+
+
+def test_bucket_columns_equal_full_build(oracle):
+    """The chunked per-column build (rows projected 32,768 at a time, only the repeats
+    the columns fall in) gives the full build's lists."""
+    rng = np.random.default_rng(41)
+    X = (rng.standard_normal((70000, 40)) * rng.lognormal(0, 0.5, (70000, 1))).astype(np.float32)
+    o = oracle.OracleCEOs(*X.shape)
+    o.setIndexParam(128, 3, 50, 4, 42)
+    o.build_coCEOs_Est(X)
+    full = o.buckets()
+    cols = np.array([0, 5, 127, 128, 200, 383])
+    part = o.bucket_columns(X, cols)
+    for a, b in zip(part, full):
+        assert np.array_equal(a, b[cols])
