@@ -514,6 +514,72 @@ hipError_t launch_clear_slot_bits(uint32_t* bits, int64_t slot0, int64_t count, 
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------
+// Centering (src/streamCEOs.cpp:22-34): the column mean of rows [0, n). Pass 1: block b
+// sums its rows [b rpb, (b+1) rpb) of every column in FP64, in row order (thread t takes
+// columns t, t + 256, ...; a wave reads 64 consecutive floats of a row). Pass 2: one
+// thread per column adds the block sums in block order and rounds sum / n to FP32 once.
+// The order is fixed, so the centre does not depend on the launch.
+__global__ __launch_bounds__(256) void k_column_partials(const float* __restrict__ X, int64_t n, int d,
+                                                         int64_t rpb, double* __restrict__ partial) {
+    const int64_t r0 = (int64_t)blockIdx.x * rpb;
+    const int64_t r1 = min(n, r0 + rpb);
+    for (int j = threadIdx.x; j < d; j += 256) {
+        double s = 0.0;
+        int64_t r = r0;
+        for (; r + 4 <= r1; r += 4) {  // four independent loads in flight, added in row order
+            const float a = X[r * d + j], b = X[(r + 1) * d + j], c = X[(r + 2) * d + j], e = X[(r + 3) * d + j];
+            s += (double)a;
+            s += (double)b;
+            s += (double)c;
+            s += (double)e;
+        }
+        for (; r < r1; ++r) s += (double)X[r * d + j];
+        partial[(int64_t)blockIdx.x * d + j] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_column_mean(const double* __restrict__ partial, int nblocks, int d,
+                                                     int64_t n, float* __restrict__ center) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= d) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += partial[(int64_t)b * d + j];
+    center[j] = (float)(s / (double)n);
+}
+
+int column_mean_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(2048, (n + 1023) / 1024)); }
+
+hipError_t launch_column_mean(const float* X, int64_t n, int d, double* partial, float* center, hipStream_t st) {
+    if (n <= 0 || d <= 0) return hipErrorInvalidValue;
+    const int nb = column_mean_blocks(n);
+    const int64_t rpb = (n + nb - 1) / nb;
+    hipLaunchKernelGGL(k_column_partials, dim3(nb), dim3(256), 0, st, X, n, d, rpb, partial);
+    hipLaunchKernelGGL(k_column_mean, dim3((d + 255) / 256), dim3(256), 0, st, partial, nb, d, n, center);
+    return hipGetLastError();
+}
+
+// x -= center (FP32) for ring slots (slot0 + i) mod cap, i < count.
+__global__ __launch_bounds__(256) void k_subtract_center(float* __restrict__ X, int64_t slot0, int64_t count,
+                                                         int64_t cap, int d, const float* __restrict__ center) {
+    const int64_t total = count * d;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+        const int64_t i = t / d;
+        const int j = (int)(t - i * d);
+        int64_t slot = slot0 + i;
+        if (slot >= cap) slot -= cap;
+        X[slot * d + j] -= center[j];
+    }
+}
+
+hipError_t launch_subtract_center(float* X, int64_t slot0, int64_t count, int64_t cap, int d, const float* center,
+                                  hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    const int grid = (int)std::min<int64_t>((count * d + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_subtract_center, dim3(grid), dim3(256), 0, st, X, slot0, count, cap, d, center);
+    return hipGetLastError();
+}
+
 // Rerank bounds: an int8 copy c of each row and (s, r1, r2, 0) per slot, one wave per
 // row. s = max_i |x_i| / 127 and c_i = round(x_i / s), clamped to [-127, 127];
 //   r1 = |x - s c|_2 + g |x|_2,  r2 = s |c|_2,  g = n u / (1 - n u), u = 2^-24, n = d + 64,

@@ -288,8 +288,31 @@ struct Core {
         note_read(s_);
     }
 
+    // Phase times of the last build / update (copy, projection and candidate scan, list
+    // selection) and of the last host-entry search (query projection + probe selection,
+    // fused search kernel), from HIP events on the stream the work ran on.
+    hipEvent_t ph_ev[5] = {};
+    ceos_phase_times last_build{}, last_search{};
+    void phase_mark(int i, hipStream_t s_) {
+        if (!ph_ev[i]) HIP_CHECK(hipEventCreate(&ph_ev[i]));
+        HIP_CHECK(hipEventRecord(ph_ev[i], s_));
+    }
+    double phase_ms(int a, int b) {
+        float ms = 0.0f;
+        HIP_CHECK(hipEventSynchronize(ph_ev[b]));
+        HIP_CHECK(hipEventElapsedTime(&ms, ph_ev[a], ph_ev[b]));
+        return ms;
+    }
+    double index_gb() const {  // the reference's figure: 4 m x DR list floats plus the stored rows
+        return (4.0 * m * DR() + (double)live * d) * 4.0 / (double)(1 << 30);
+    }
+
     Core() = default;
     ~Core() {
+        for (hipEvent_t e : ph_ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : sev)
+            if (e) (void)hipEventDestroy(e);
         for (auto& x : rd_ev) {
             if (x.pending) (void)hipEventSynchronize(x.e);
             (void)hipEventDestroy(x.e);
@@ -340,6 +363,8 @@ struct Core {
 
     void clear() {
         wait_reads();
+        dcenter.release();
+        center.clear();
         bkt.release(); bsize.release(); X.release(); signs.release(); dead.release();
         Xq.release(); Xqs.release();
         scr.release(); estP.release();
@@ -377,6 +402,20 @@ struct Core {
         cap = capacity; head = 0; base = 0; live = (uint32_t)n;
         HIP_CHECK(hipMemcpyAsync(X.p, src, (size_t)n * d * 4,
                                  src_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s_));
+    }
+
+    // Centering (src/streamCEOs.cpp:22-34): the column mean of rows 0..n-1 of X (ring slots
+    // 0..n-1) is computed on the device, subtracted from those rows, kept on the device
+    // for later inserts and copied to `center` on the host (index files, get_center).
+    DBuf<float> dcenter;
+    void center_rows(int64_t n, hipStream_t s_) {
+        DBuf<double> partial((size_t)ceos::column_mean_blocks(n) * d);
+        dcenter.alloc(d);
+        HIP_CHECK(ceos::launch_column_mean(X.p, n, d, partial.p, dcenter.p, s_));
+        HIP_CHECK(ceos::launch_subtract_center(X.p, 0, n, cap, d, dcenter.p, s_));
+        center.resize(d);
+        HIP_CHECK(hipMemcpyAsync(center.data(), dcenter.p, (size_t)d * 4, hipMemcpyDeviceToHost, s_));
+        HIP_CHECK(hipStreamSynchronize(s_));  // partial is freed on return
     }
 
     // coCEOs buckets over rows 0..n-1 of X (ring slots 0..n-1): see ceos_build.hip.
@@ -450,8 +489,18 @@ struct Core {
         }
         bkt.alloc((size_t)nl * m);
         bsize.alloc(nl);
+        phase_mark(2, s_);
         HIP_CHECK(ceos::launch_list_select(nl, m, dbase.p, dcap.p, dcount.p, ent.p, bkt.p, bsize.p, s_));
+        phase_mark(3, s_);
         HIP_CHECK(hipStreamSynchronize(s_));
+    }
+    // Phase record of a build that marked 0 / 1 around its copy and ran build_buckets.
+    void finish_build_times(std::chrono::steady_clock::time_point t0) {
+        last_build.copy_ms = phase_ms(0, 1);
+        last_build.project_ms = phase_ms(1, 2);
+        last_build.select_ms = phase_ms(2, 3);
+        last_build.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        last_build.index_gb = index_gb();
     }
 
     // Picks the LDS score-tile size and grid for a search with the given bounds:
@@ -567,11 +616,17 @@ struct Core {
     // Uniform-parameter coCEOs search on device pointers.
     // Uniform-parameter search on device pointers; hash = CEOs-Hash (every distinct
     // candidate reranked, c unused).
+    hipEvent_t sev[3] = {};  // host-entry searches: before the front kernels, before and after the fused kernel
     void search_device(const float* dQ, int nq, int k, int s_, int c_, int32_t* ids, float* ips, hipStream_t s__,
-                       bool hash = false) {
+                       bool hash = false, bool phase_events = false) {
         if (nq <= 0 || k == 0) return;
         if (hash) c_ = 1;
-        const ceos::QueryConfig cfg = search_config(s_, c_, k);
+        ceos::QueryConfig cfg = search_config(s_, c_, k);
+        if (phase_events && !cfg.ev[0]) {
+            for (auto& e : sev)
+                if (!e) HIP_CHECK(hipEventCreate(&e));
+            for (int i = 0; i < 3; ++i) cfg.ev[i] = sev[i];
+        }
         before_scratch(s__);
         const ceos::QueryScratch scr = scratch(cfg, nq);
         ceos::QueryBatch qb{dQ, nq, nullptr, nullptr, nullptr, k, s_, c_, mp, nullptr, ids, ips, nullptr, hash ? 1 : 0};
@@ -581,15 +636,26 @@ struct Core {
 
     void search_host(const float* Q, int nq, int k, int s_, int c_, int32_t* ids, float* ips, bool hash = false) {
         if (nq <= 0 || k == 0) return;
+        const auto t0 = std::chrono::steady_clock::now();
         DBuf<float> dQ((size_t)nq * d);
         DBuf<int32_t> dI((size_t)nq * k);
         DBuf<float> dP((size_t)nq * k);
         HIP_CHECK(hipMemcpyAsync(dQ.p, Q, (size_t)nq * d * 4, hipMemcpyHostToDevice, st()));
-        search_device(dQ.p, nq, k, s_, c_, dI.p, dP.p, st(), hash);
+        search_device(dQ.p, nq, k, s_, c_, dI.p, dP.p, st(), hash, !ktiming);
         HIP_CHECK(hipMemcpyAsync(ids, dI.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, st()));
         HIP_CHECK(hipMemcpyAsync(ips, dP.p, (size_t)nq * k * 4, hipMemcpyDeviceToHost, st()));
         HIP_CHECK(hipStreamSynchronize(st()));
         check_status();
+        last_search = ceos_phase_times{};
+        if (!ktiming && sev[2]) {
+            float a = 0.0f, b = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&a, sev[0], sev[1]));
+            HIP_CHECK(hipEventElapsedTime(&b, sev[1], sev[2]));
+            last_search.project_ms = a;
+            last_search.select_ms = b;
+        }
+        last_search.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        last_search.index_gb = index_gb();
     }
 
     // ---- index files (no reference counterpart): parameters, points in logical order,
@@ -720,6 +786,11 @@ struct Core {
         seed = (int)(seed_v & 0x7fffffffu) == (int)seed_v ? (int)seed_v : -1;
         params_set = true;
         center = std::move(ctr);
+        dcenter.release();
+        if (!center.empty()) {  // a stream's centre, subtracted from later inserts
+            dcenter.alloc(d);
+            HIP_CHECK(hipMemcpy(dcenter.p, center.data(), (size_t)d * 4, hipMemcpyHostToDevice));
+        }
         X.alloc((size_t)std::max<int64_t>(n, 1) * d);
         cap = std::max<int64_t>(n, 1); head = 0; base = 0; live = (uint32_t)n;
         const int64_t chunk = std::max<int64_t>(1, (int64_t(64) << 20) / (4LL * d));

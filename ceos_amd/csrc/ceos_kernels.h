@@ -154,6 +154,14 @@ int est_search_lds_bytes(const IndexDims& dm, int s_max, int c_max, int k_max);
 
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 
+// Centering: center[j] = FP32 of the FP64 mean of column j over rows [0, n) of X, summed
+// in a fixed order (partial: column_mean_blocks(n) x d doubles of workspace).
+int column_mean_blocks(int64_t n);
+hipError_t launch_column_mean(const float* X, int64_t n, int d, double* partial, float* center, hipStream_t st);
+// X[(slot0 + i) mod cap][j] -= center[j] for i < count (count <= cap).
+hipError_t launch_subtract_center(float* X, int64_t slot0, int64_t count, int64_t cap, int d, const float* center,
+                                  hipStream_t st);
+
 // Clears bit (slot0 + j) mod cap of a slot bitmap for j < count (count <= cap).
 hipError_t launch_clear_slot_bits(uint32_t* bits, int64_t slot0, int64_t count, int64_t cap, hipStream_t st);
 

@@ -38,6 +38,16 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// The search phase lines printed when verbose (src/CEOs.cpp:676-680): the device runs
+// score accumulation, candidate selection and the rerank as one fused kernel, so they
+// share one line.
+void print_search_phases(const ceos_phase_times& t, const char* proj_label) {
+    std::printf("%s: %g ms\n", proj_label, t.project_ms);
+    std::printf("Estimating, extracting candidates and computing distance time (one fused kernel): %g ms\n",
+                t.select_ms);
+    std::printf("Querying time: %g ms\n", t.total_ms);
+}
+
 py::tuple result(int nq, int k, py::array_t<int32_t>& ids, py::array_t<float>& ips) {
     (void)nq; (void)k;
     return py::make_tuple(ids, ips);
@@ -125,12 +135,21 @@ class PyCEOs {
     void set_threads(int t) { check(ceos_set_threads(h_, t)); }
     void clear() { check(ceos_clear(h_)); }
 
-    void print_params(const char* title) const {
+    void print_params(const char* title, bool top_m = true) const {
         int F, D, R, m, seed;
         check(ceos_get_dims(h_, &F, &D, &R, &m, &seed));
-        std::printf("%s\nn_points: %d\nn_features: %d\nn_repeats: %d\nn_proj: %d\ntop_m: %d\nfhtDim: %d\n", title, n_,
-                    d_, R, D, m, F);
+        std::printf("%s\nn_points: %d\nn_features: %d\nn_repeats: %d\nn_proj: %d\n", title, n_, d_, R, D);
+        if (top_m) std::printf("top_m: %d\n", m);
+        std::printf("fhtDim: %d\n", F);
     }
+    int threads() const { int t = 0; check(ceos_get_threads(h_, &t)); return t; }
+    py::array_t<float> center() const {
+        py::array_t<float> out(d_);
+        check(ceos_get_center(h_, out.mutable_data()));
+        return out;
+    }
+    ceos_phase_times build_times() const { ceos_phase_times t{}; check(ceos_get_build_times(h_, &t)); return t; }
+    ceos_phase_times search_times() const { ceos_phase_times t{}; check(ceos_get_search_times(h_, &t)); return t; }
 
     void build_coCEOs_Est(FArr X) {
         require_2d(X, "dataset");
@@ -142,6 +161,11 @@ class PyCEOs {
             rc = ceos_build_coceos(h_, X.data(), X.shape(0), (int)X.shape(1));
         }
         check(rc);
+        const ceos_phase_times t = build_times();  // src/CEOs.cpp:326-328, 434-440
+        std::printf("Copying data time (in seconds): %g\n", t.copy_ms / 1000.0);
+        std::printf("Size of coCEOs-Est index in GB: %g\n", t.index_gb);
+        std::printf("Projecting time: %g ms\n", t.project_ms);
+        std::printf("Extracting top-m points time: %g ms\n", t.select_ms);
         std::printf("Constructing time (in seconds): %.3f\n", ms_since(t0) / 1000.0);
         std::fflush(stdout);
     }
@@ -156,6 +180,14 @@ class PyCEOs {
             rc = ceos_build_hash(h_, X.data(), X.shape(0), (int)X.shape(1));
         }
         check(rc);
+        const ceos_phase_times t = build_times();  // src/CEOs.cpp:731, 834-839
+        int F, D, R, m, seed;
+        check(ceos_get_dims(h_, &F, &D, &R, &m, &seed));
+        std::printf("Copying and centering data time in seconds: %g\n", t.copy_ms / 1000.0);
+        std::printf("Size of CEOs-Hash index in GB: %g\n",
+                    (2.0 * m * D * R + (double)X.shape(0) * d_) * 4.0 / (double)(1 << 30));
+        std::printf("Projecting time: %g ms\n", t.project_ms);
+        std::printf("Extracting top-points time: %g ms\n", t.select_ms);
         std::printf("Constructing time (in seconds): %.3f\n", ms_since(t0) / 1000.0);
         std::fflush(stdout);
     }
@@ -173,9 +205,10 @@ class PyCEOs {
             rc = ceos_search_hash(h_, Q.data(), nq, (int)Q.shape(1), k, ids.mutable_data(), ips.mutable_data());
         }
         check(rc);
-        if (verbose) {
-            std::printf("n_queries: %d\nn_probedVectors: %d\nn_probedPoints: %d\nQuerying time: %.3f ms\n", nq, get_s(),
-                        get_mp(), ms_since(t0));
+        if (verbose) {  // src/CEOs.cpp:866-870, 1047-1052
+            std::printf("n_queries: %d\nn_probedVectors: %d\nn_probedPoints: %d\nn_threads: %d\n", nq, get_s(), get_mp(),
+                        threads());
+            print_search_phases(search_times(), "Projecting and extracting top-vectors time");
             std::fflush(stdout);
         }
         return result(nq, k, ids, ips);
@@ -194,9 +227,10 @@ class PyCEOs {
             rc = ceos_search_coceos(h_, Q.data(), nq, (int)Q.shape(1), k, ids.mutable_data(), ips.mutable_data());
         }
         check(rc);
-        if (verbose) {
-            std::printf("n_queries: %d\nn_probedVectors: %d\nn_probedPoints: %d\nn_cand: %d\nQuerying time: %.3f ms\n", nq,
-                        get_s(), get_mp(), get_c(), ms_since(t0));
+        if (verbose) {  // src/CEOs.cpp:466-470, 676-680
+            std::printf("n_queries: %d\nn_probedVectors: %d\nn_probedPoints: %d\nn_cand: %d\nn_threads: %d\n", nq,
+                        get_s(), get_mp(), get_c(), threads());
+            print_search_phases(search_times(), "Projecting and extracting top-vectors time");
             std::fflush(stdout);
         }
         return result(nq, k, ids, ips);
@@ -251,7 +285,7 @@ class PyCEOs {
 
     void build(FArr X) {
         require_2d(X, "dataset");
-        print_params("Building CEOs index...");
+        print_params("Building CEOs index...", false);
         const auto t0 = std::chrono::steady_clock::now();
         int rc;
         {
@@ -259,6 +293,12 @@ class PyCEOs {
             rc = ceos_build_est(h_, X.data(), X.shape(0), (int)X.shape(1));
         }
         check(rc);
+        const ceos_phase_times t = build_times();  // src/CEOs.cpp:47, 91-97
+        int F, D, R, m, seed;
+        check(ceos_get_dims(h_, &F, &D, &R, &m, &seed));
+        std::printf("Copying data time (in seconds): %g\n", t.copy_ms / 1000.0);
+        std::printf("Size of CEOs index in GB: %g\n", (double)X.shape(0) * D * R * 4.0 / (double)(1 << 30));
+        std::printf("Size of data set in GB: %g\n", (double)X.shape(0) * d_ * 4.0 / (double)(1 << 30));
         std::printf("Index construction time (in seconds): %.3f\n", ms_since(t0) / 1000.0);
         std::fflush(stdout);
     }
@@ -276,9 +316,10 @@ class PyCEOs {
             rc = ceos_search_est(h_, Q.data(), nq, (int)Q.shape(1), k, ids.mutable_data(), ips.mutable_data());
         }
         check(rc);
-        if (verbose) {
-            std::printf("n_queries: %d\nn_probed_vectors: %d\nn_cand: %d\nQuerying time: %.3f ms\n", nq, get_s(),
-                        get_c(), ms_since(t0));
+        if (verbose) {  // src/CEOs.cpp:113-117, 280-284 (one fused kernel: the total only)
+            std::printf("n_queries: %d\nn_probed_vectors: %d\nn_probed_points: %d\nn_cand: %d\nn_threads: %d\n", nq,
+                        get_s(), get_mp(), get_c(), threads());
+            std::printf("Querying time: %g ms\n", ms_since(t0));
             std::fflush(stdout);
         }
         return result(nq, k, ids, ips);
@@ -357,6 +398,17 @@ class PyStream {
         check(rc);
     }
     int dim() const { return d_; }
+    int threads() const { int t = 0; check(ceos_stream_get_threads(h_, &t)); return t; }
+    ceos_phase_times search_times() const {
+        ceos_phase_times t{};
+        check(ceos_stream_get_search_times(h_, &t));
+        return t;
+    }
+    py::array_t<float> center() const {
+        py::array_t<float> out(d_);
+        check(ceos_stream_get_center(h_, out.mutable_data()));
+        return out;
+    }
 
     void save(const std::string& path) const {
         int rc;
@@ -402,8 +454,17 @@ class PyStream {
             rc = ceos_stream_build(h_, X.data(), X.shape(0), (int)X.shape(1));
         }
         check(rc);
-        std::printf("n_features: %d\nn_points: %lld\nConstructing time (in seconds): %.3f\n", d_,
-                    (long long)X.shape(0), ms_since(t0) / 1000.0);
+        int F, D, R, m, seed;
+        check(ceos_stream_get_dims(h_, &F, &D, &R, &m, &seed));
+        ceos_phase_times t{};
+        check(ceos_stream_get_build_times(h_, &t));  // src/streamCEOs.cpp:11-20, 37, 154-160
+        std::printf("n_features: %d\nn_proj: %d\ntop_m: %d\nfhtDim: %d\ncentering: %d\nn_points: %lld\n", d_, D, m, F,
+                    get_centering() ? 1 : 0, (long long)X.shape(0));
+        std::printf("Copying data time (in ms): %g ms\n", t.copy_ms);
+        std::printf("Size of coCEOs-Est index in GB: %g\n", t.index_gb);
+        std::printf("Projecting time (in ms): %g ms\n", t.project_ms);
+        std::printf("Extracting top-points time (in ms): %g ms\n", t.select_ms);
+        std::printf("Constructing time (in seconds): %.3f seconds\n", ms_since(t0) / 1000.0);
         std::fflush(stdout);
     }
     void update(FArr newX, int n_del) {
@@ -414,6 +475,15 @@ class PyStream {
             rc = ceos_stream_update(h_, newX.data(), newX.shape(0), (int)newX.shape(1), n_del);
         }
         check(rc);
+        ceos_phase_times t{};
+        check(ceos_stream_get_build_times(h_, &t));  // src/streamCEOs.cpp:212-215, 362-368
+        std::printf("n_points (after update): %lld\n", (long long)size());
+        std::printf("Updating data time (in ms): %g ms\n", t.copy_ms);
+        std::printf("Size of coCEOs-Est index in GB: %g\n", t.index_gb);
+        std::printf("Projecting time (in ms): %g ms\n", t.project_ms);
+        std::printf("Extracting top-points time (in ms): %g ms\n", t.select_ms);
+        std::printf("Updating time (in ms): %g ms\n", t.total_ms);
+        std::fflush(stdout);
     }
     void delete_ids(IArr ids) {
         int rc;
@@ -437,9 +507,10 @@ class PyStream {
                                              ips.mutable_data());
         }
         check(rc);
-        if (verbose) {
-            std::printf("n_queries: %d\nn_probedVectors: %d\nn_cand: %d\nQuerying time: %.3f ms\n", nq, get_s(), get_c(),
-                        ms_since(t0));
+        (void)t0;
+        if (verbose) {  // src/streamCEOs.cpp:396-399, 594-598
+            std::printf("n_queries: %d\nn_probedVectors: %d\nn_cand: %d\nn_threads: %d\n", nq, get_s(), get_c(), threads());
+            print_search_phases(search_times(), "Projecting time");
             std::fflush(stdout);
         }
         return py::make_tuple(ids, ips);
@@ -459,9 +530,11 @@ class PyStream {
                                              ips.mutable_data());
         }
         check(rc);
-        if (verbose) {
-            std::printf("n_queries: %d\nn_probedVectors: %d\nn_cand: %d\nQuerying time: %.3f ms\n", nq, get_s(), get_c(),
-                        ms_since(t0));
+        (void)t0;
+        if (verbose) {  // src/streamCEOs.cpp:638-642, 858-863
+            std::printf("n_queries: %d\nn_probedVectors: %d\nn_probedPoints: %d\nn_cand: %d\nn_threads: %d\n", nq, get_s(),
+                        get_mp(), get_c(), threads());
+            print_search_phases(search_times(), "Projecting time");
             std::fflush(stdout);
         }
         return py::make_tuple(ids, ips);
@@ -614,6 +687,7 @@ PYBIND11_MODULE(CEOs, m) {
         .def_property("n_probed_points", &PyCEOs::get_mp, &PyCEOs::set_mp)
         .def_property("n_cand", &PyCEOs::get_c, &PyCEOs::set_c)
         .def_property("centering", &PyCEOs::get_centering, &PyCEOs::set_centering)
+        .def_property_readonly("center", &PyCEOs::center)
         .def_property("rerank_bounds", &PyCEOs::get_rerank_bounds, &PyCEOs::set_rerank_bounds)
         .def("set_threads", &PyCEOs::set_threads, py::arg("n_threads"))
         .def("clear", &PyCEOs::clear)
@@ -680,6 +754,7 @@ PYBIND11_MODULE(CEOs, m) {
         .def_property("n_probed_points", &PyStream::get_mp, &PyStream::set_mp)
         .def_property("n_cand", &PyStream::get_c, &PyStream::set_c)
         .def_property("centering", &PyStream::get_centering, &PyStream::set_centering)
+        .def_property_readonly("center", &PyStream::center)
         .def_property("rerank_bounds", &PyStream::get_rerank_bounds, &PyStream::set_rerank_bounds)
         .def("set_threads", &PyStream::set_threads, py::arg("n_threads"))
         .def("clear", &PyStream::clear)

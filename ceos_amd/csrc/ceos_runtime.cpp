@@ -78,21 +78,30 @@ int ceos_get_dims(const ceos_index* h, int* fht, int* n_proj, int* n_repeats, in
     });
 }
 
-static void build_coceos_impl(ceos_index* h, const float* X, int64_t n, int d, bool dev, hipStream_t st) {
+// center: subtract the points' column mean on the device after the copy (CEOs-Hash).
+static void build_coceos_impl(ceos_index* h, const float* X, int64_t n, int d, bool dev, hipStream_t st,
+                              bool center = false) {
     Core& c = h->core;
     if (!c.params_set) throw StateError("setIndexParam must be called before build");
     check_matrix(X, n, d, c.d);
     if (n <= 0) throw ArgError("empty dataset");
     if (n >= (int64_t(1) << 31)) throw ArgError("more than 2^31-1 points");
     c.wait_reads();
+    const auto t0 = std::chrono::steady_clock::now();
     c.built = false;
     h->n_points = (int)n;
     c.estP.release();
+    c.dcenter.release();
+    c.center.clear();
+    c.phase_mark(0, st);
     c.load_points(X, n, n, dev, st);
+    if (center) c.center_rows(n, st);
+    c.phase_mark(1, st);
     c.make_signs();
     c.build_buckets(n, st);
     c.quantize_rows(0, n, n, st);
     HIP_CHECK(hipStreamSynchronize(st));
+    c.finish_build_times(t0);
     c.built = true;
 }
 
@@ -122,17 +131,7 @@ int ceos_build_hash(ceos_index* h, const float* X, int64_t n, int d) {
         std::lock_guard<std::mutex> g(h->core.mu);
         Core& c = h->core;
         check_matrix(X, n, d, c.d);
-        if (!c.centering) {
-            build_coceos_impl(h, X, n, d, false, c.st());
-            return;
-        }
-        std::vector<double> mean(d, 0.0);
-        for (int64_t i = 0; i < n; ++i)
-            for (int j = 0; j < d; ++j) mean[j] += X[i * d + j];
-        std::vector<float> Xc((size_t)n * d);
-        for (int64_t i = 0; i < n; ++i)
-            for (int j = 0; j < d; ++j) Xc[i * d + j] = X[i * d + j] - (float)(mean[j] / (double)n);
-        build_coceos_impl(h, Xc.data(), n, d, false, c.st());
+        build_coceos_impl(h, X, n, d, false, c.st(), c.centering);
     });
 }
 
@@ -268,7 +267,11 @@ int ceos_build_est(ceos_index* h, const float* X, int64_t n, int d) {
         h->n_points = (int)n;
         c.bkt.release();
         c.bsize.release();
+        c.dcenter.release();
+        c.center.clear();
+        c.phase_mark(0, c.st());
         c.load_points(X, n, n, false, c.st());
+        c.phase_mark(1, c.st());
         c.make_signs();
         c.estP.alloc((size_t)c.DR() * n);
         c.est_n = n;
@@ -284,7 +287,11 @@ int ceos_build_est(ceos_index* h, const float* X, int64_t n, int d) {
                                                 c.st()));
             HIP_CHECK(ceos::launch_transpose(tmp.p, rows, DR_, c.estP.p + r0, c.st(), n));
         }
+        c.phase_mark(2, c.st());
         HIP_CHECK(hipStreamSynchronize(c.st()));
+        c.last_build = ceos_phase_times{};
+        c.last_build.copy_ms = c.phase_ms(0, 1);
+        c.last_build.project_ms = c.phase_ms(1, 2);
         c.built = true;
     });
 }
@@ -530,6 +537,20 @@ int ceos_stream_get_centering(const ceos_stream* h, int* centering) {
 int ceos_stream_set_threads(ceos_stream* h, int n_threads) {
     return guarded([&] { if (!h) throw ArgError("null handle"); h->core.threads = n_threads; });
 }
+int ceos_stream_get_threads(const ceos_stream* h, int* n_threads) {
+    return guarded([&] { if (!h || !n_threads) throw ArgError("null argument"); *n_threads = h->core.threads; });
+}
+int ceos_stream_get_dims(const ceos_stream* h, int* fht, int* n_proj, int* n_repeats, int* top_m, int* seed) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        const Core& c = h->core;
+        if (fht) *fht = c.F;
+        if (n_proj) *n_proj = c.D;
+        if (n_repeats) *n_repeats = c.R;
+        if (top_m) *top_m = c.m;
+        if (seed) *seed = (int)c.seed_used;
+    });
+}
 
 int ceos_stream_clear(ceos_stream* h) {
     return guarded([&] {
@@ -549,30 +570,24 @@ int ceos_stream_build(ceos_stream* h, const float* X, int64_t n, int d) {
         if (n <= 0) throw ArgError("empty dataset");
         if (n >= (int64_t(1) << 31)) throw ArgError("more than 2^31-1 points");
         c.wait_reads();
+        const auto t0 = std::chrono::steady_clock::now();
         c.built = false;
         const int64_t capacity = std::max<int64_t>(n + n / 4, 1024);
+        c.dcenter.release();
+        c.center.clear();
+        c.phase_mark(0, c.st());
+        c.load_points(X, n, capacity, false, c.st());
         // centering: rows are stored minus the build rows' column mean, which stays
         // fixed for later updates (src/streamCEOs.cpp:22-34, 198-203)
-        std::vector<float> Xc;
-        c.center.clear();
-        if (c.centering) {
-            std::vector<double> mean(d, 0.0);
-            for (int64_t i = 0; i < n; ++i)
-                for (int j = 0; j < d; ++j) mean[j] += X[i * d + j];
-            c.center.resize(d);
-            for (int j = 0; j < d; ++j) c.center[j] = (float)(mean[j] / (double)n);
-            Xc.resize((size_t)n * d);
-            for (int64_t i = 0; i < n; ++i)
-                for (int j = 0; j < d; ++j) Xc[i * d + j] = X[i * d + j] - c.center[j];
-            X = Xc.data();
-        }
-        c.load_points(X, n, capacity, false, c.st());
+        if (c.centering) c.center_rows(n, c.st());
+        c.phase_mark(1, c.st());
         c.dead.alloc((size_t)(capacity + 31) / 32);
         HIP_CHECK(hipMemsetAsync(c.dead.p, 0, c.dead.n * 4, c.st()));
         c.make_signs();
         c.build_buckets(n, c.st());
         c.quantize_rows(0, n, n, c.st());
         HIP_CHECK(hipStreamSynchronize(c.st()));
+        c.finish_build_times(t0);
         c.built = true;
     });
 }
@@ -588,15 +603,9 @@ int ceos_stream_update(ceos_stream* h, const float* Xn, int64_t n_new, int d, in
         if (n_del < 0) throw ArgError("n_delPoints must be >= 0");
         if (n_del > n_cur) throw ArgError("Number of removed points must be smaller than the current number of points");
         if (c.m > n_cur + n_new - n_del) throw ArgError("There is not enough indexed top-points for coCEOs after update");
-        std::vector<float> Xc;
-        if (c.centering && !c.center.empty() && n_new > 0) {  // src/streamCEOs.cpp:198-203
-            Xc.resize((size_t)n_new * d);
-            for (int64_t i = 0; i < n_new; ++i)
-                for (int j = 0; j < d; ++j) Xc[i * d + j] = Xn[i * d + j] - c.center[j];
-            Xn = Xc.data();
-        }
         if (n_cur + n_new >= (int64_t(1) << 31)) throw ArgError("more than 2^31-1 live points");
         c.wait_reads();
+        const auto t0 = std::chrono::steady_clock::now();
         hipStream_t st = c.st();
         // grow the ring when the new rows do not fit
         if (n_cur + n_new > c.cap) {
@@ -632,11 +641,18 @@ int ceos_stream_update(ceos_stream* h, const float* Xn, int64_t n_new, int d, in
         int64_t slot0 = c.head + n_cur;
         if (slot0 >= c.cap) slot0 -= c.cap;
         const int64_t seg1 = std::min<int64_t>(n_new, c.cap - slot0);
+        c.phase_mark(0, st);
         if (n_new > 0) {
             HIP_CHECK(hipMemcpyAsync(c.X.p + slot0 * c.d, Xn, (size_t)seg1 * c.d * 4, hipMemcpyHostToDevice, st));
             if (n_new > seg1)
                 HIP_CHECK(hipMemcpyAsync(c.X.p, Xn + seg1 * c.d, (size_t)(n_new - seg1) * c.d * 4,
                                          hipMemcpyHostToDevice, st));
+            // centering: new rows minus the build's centre (src/streamCEOs.cpp:198-203)
+            if (c.centering && c.dcenter.p)
+                HIP_CHECK(ceos::launch_subtract_center(c.X.p, slot0, n_new, c.cap, c.d, c.dcenter.p, st));
+        }
+        c.phase_mark(1, st);
+        if (n_new > 0) {
             if (c.dead.p) HIP_CHECK(ceos::launch_clear_slot_bits(c.dead.p, slot0, n_new, c.cap, st));
             c.quantize_rows(slot0, n_new, n_cur + n_new, st);
         }
@@ -661,12 +677,15 @@ int ceos_stream_update(ceos_stream* h, const float* Xn, int64_t n_new, int d, in
         }
         const uint32_t new_base = c.base + (uint32_t)n_del;
         const uint32_t new_live = (uint32_t)(n_cur + n_new - n_del);
+        c.phase_mark(2, st);
         HIP_CHECK(ceos::launch_stream_merge(c.DR(), c.m, c.bkt.p, c.bsize.p, np.p, n_new, first_abs, new_base,
                                             new_live, st));
+        c.phase_mark(3, st);
         HIP_CHECK(hipStreamSynchronize(st));
         c.base = new_base;
         c.live = new_live;
         c.head = (c.head + n_del) % c.cap;
+        c.finish_build_times(t0);
     });
 }
 
@@ -748,6 +767,47 @@ int ceos_stream_get_buckets(const ceos_stream* h, int32_t* cid, float* cval, int
         std::lock_guard<std::mutex> g(c.mu);
         c.get_buckets(cid, cval, fid, fval, csz, fsz);
     });
+}
+
+// ------------------------- phase times and the centre -----------------------------
+
+static int phase_times(const Core& c, bool search, ceos_phase_times* out) {
+    return guarded([&] {
+        if (!out) throw ArgError("null argument");
+        *out = search ? c.last_search : c.last_build;
+    });
+}
+int ceos_get_build_times(const ceos_index* h, ceos_phase_times* out) {
+    if (!h) { g_err = "null handle"; return CEOS_EINVAL; }
+    return phase_times(h->core, false, out);
+}
+int ceos_get_search_times(const ceos_index* h, ceos_phase_times* out) {
+    if (!h) { g_err = "null handle"; return CEOS_EINVAL; }
+    return phase_times(h->core, true, out);
+}
+int ceos_stream_get_build_times(const ceos_stream* h, ceos_phase_times* out) {
+    if (!h) { g_err = "null handle"; return CEOS_EINVAL; }
+    return phase_times(h->core, false, out);
+}
+int ceos_stream_get_search_times(const ceos_stream* h, ceos_phase_times* out) {
+    if (!h) { g_err = "null handle"; return CEOS_EINVAL; }
+    return phase_times(h->core, true, out);
+}
+
+static int get_center(const Core& c, float* out) {
+    return guarded([&] {
+        if (!out) throw ArgError("null argument");
+        if (c.center.empty()) throw StateError("no centre: centering was off at the last build");
+        std::memcpy(out, c.center.data(), c.center.size() * 4);
+    });
+}
+int ceos_get_center(const ceos_index* h, float* out) {
+    if (!h) { g_err = "null handle"; return CEOS_EINVAL; }
+    return get_center(h->core, out);
+}
+int ceos_stream_get_center(const ceos_stream* h, float* out) {
+    if (!h) { g_err = "null handle"; return CEOS_EINVAL; }
+    return get_center(h->core, out);
 }
 
 // ------------------------------- search plans ------------------------------------

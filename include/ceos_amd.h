@@ -37,6 +37,17 @@ extern "C" {
 typedef struct ceos_index ceos_index;
 typedef struct ceos_stream ceos_stream;
 
+/* Phase times of a build, update or search, in milliseconds of device time (HIP events),
+ * and the index size in GB by the reference's formula (4 top_m x n_proj*n_repeats list
+ * floats plus the stored rows). Builds and updates: copy = the points into HBM; project =
+ * projections and candidate scan; select = exact top-m per list (or the stream merge).
+ * Searches: project = query projection + probe selection; select = the fused kernel
+ * (score accumulation, candidate selection, rerank, top-k). */
+typedef struct {
+    double copy_ms, project_ms, select_ms, total_ms;
+    double index_gb;
+} ceos_phase_times;
+
 /* Description of the last failure on this thread ("" if none). */
 const char* ceos_last_error(void);
 
@@ -165,6 +176,23 @@ int ceos_get_kernel_timing(ceos_index* h, double* front_ms, double* fused_ms, in
 int ceos_read_bandwidth(const float* x, int64_t n, float* out, int64_t n_out, int iters, void* stream,
                         double* gbs);
 
+/* Phase times of the last build (or stream update) and of the last host-entry search.
+ * The reference prints these instead of returning them: "Copying data time", "Size of
+ * coCEOs-Est index in GB", "Projecting time", "Extracting top-m points time",
+ * "Constructing time" (src/CEOs.cpp:326-328, 434-440; src/streamCEOs.cpp:36-37, 151-159,
+ * 210-221, 355-368) and, verbose, the search's phase lines (src/CEOs.cpp:674-681). */
+int ceos_get_build_times(const ceos_index* h, ceos_phase_times* out);
+int ceos_get_search_times(const ceos_index* h, ceos_phase_times* out);
+int ceos_stream_get_build_times(const ceos_stream* h, ceos_phase_times* out);
+int ceos_stream_get_search_times(const ceos_stream* h, ceos_phase_times* out);
+
+/* The centre subtracted from the stored points (n_features floats) when the last build
+ * had centering on: CEOs-Hash (src/CEOs.cpp:721-728) and streamCEOs
+ * (src/streamCEOs.cpp:22-25, vec_center). Computed on the device as the FP64 column mean
+ * of the build rows, rounded once to FP32. CEOS_ESTATE when there is none. */
+int ceos_get_center(const ceos_index* h, float* out);
+int ceos_stream_get_center(const ceos_stream* h, float* out);
+
 /* hipStream_t the index launches on (as void*). */
 void* ceos_get_stream(const ceos_index* h);
 
@@ -271,6 +299,9 @@ int ceos_stream_set_query_param(ceos_stream* h, int n_probed_vectors, int n_prob
 int ceos_stream_get_query_param(const ceos_stream* h, int* n_probed_vectors, int* n_probed_points,
                                 int* n_cand);
 int ceos_stream_set_threads(ceos_stream* h, int n_threads);
+int ceos_stream_get_threads(const ceos_stream* h, int* n_threads);
+/* fhtDim, n_proj, n_repeats, top_m and the seed used (as ceos_get_dims). */
+int ceos_stream_get_dims(const ceos_stream* h, int* fht_dim, int* n_proj, int* n_repeats, int* top_m, int* seed);
 /* centering (streamCEOs.h:44, not bound in python_wrapper.cpp:52-71): stored rows are
  * minus the build rows' column mean, fixed for later updates (streamCEOs.cpp:22-34,
  * 198-203); queries are not centered. Takes effect at the next build. */

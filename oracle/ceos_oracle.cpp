@@ -138,6 +138,10 @@ void build_buckets(const Params& P, const std::vector<float>& proj, int64_t n, B
         MinHeap close, far;
         for (int64_t i = 0; i < n; ++i) {
             const float v = proj[(size_t)i * DR + c];
+            // A NaN never replaces a heap entry under the strict '>' (src/CEOs.cpp:395,406);
+            // while a heap is still filling, the reference would admit one and lose the
+            // heap order from then on. The defined rule used here: NaN never enters a list.
+            if (std::isnan(v)) continue;
             if ((int)close.size() < P.m) close.emplace((int)i, v);
             else if (v > close.top().v) { close.pop(); close.emplace((int)i, v); }
             if ((int)far.size() < P.m) far.emplace((int)i, -v);
@@ -290,7 +294,29 @@ struct OracleStream {
     Params P;
     std::deque<std::vector<float>> X;  // src/streamCEOs.h:26
     Buckets B;
+    bool centering = false;            // src/streamCEOs.h:44
+    std::vector<float> center;         // vec_center (src/streamCEOs.cpp:24-25)
+    std::vector<float> given_center;   // when set: used by the next build instead of the mean
 };
+
+// matX.colwise().mean() of a row-major FP32 matrix (src/streamCEOs.cpp:24-25), in the
+// order Eigen 3.4's packet-wise partial reduction (PartialReduxEvaluator.h) sums a
+// column: p = x_0, then p = p + ((x_i + x_{i+1}) + (x_{i+2} + x_{i+3})) over groups of
+// four rows while i < (n - 1) & ~3, then the remaining rows one at a time; the sum is
+// divided by n in FP32. Eigen is not part of the reference tree and its version is not
+// pinned, so this order is one plausible evaluation; the tests bound the result against
+// the FP64 mean instead of relying on it bit for bit.
+static void column_mean_eigen(const float* X, int64_t n, int d, float* out) {
+    for (int j = 0; j < d; ++j) {
+        auto x = [&](int64_t i) { return X[(size_t)i * d + j]; };
+        float p = x(0);
+        const int64_t size4 = (n - 1) & ~int64_t(3);
+        int64_t i = 1;
+        for (; i < size4; i += 4) p = p + ((x(i) + x(i + 1)) + (x(i + 2) + x(i + 3)));
+        for (; i < n; ++i) p = p + x(i);
+        out[j] = p / (float)n;
+    }
+}
 
 extern "C" {
 
@@ -404,6 +430,7 @@ void oracle_ceos_bucket_columns(void* hv, const float* X, int64_t n, const int* 
             const size_t ri = std::find(reps.begin(), reps.end(), r) - reps.begin();
             for (int64_t i = a; i < b; ++i) {
                 const float v = proj[((size_t)(i - a) * reps.size() + ri) * P.D + j];
+                if (std::isnan(v)) continue;  // NaN never enters a list (as build_buckets)
                 if ((int)close[ci].size() < P.m) close[ci].emplace((int)i, v);
                 else if (v > close[ci].top().v) { close[ci].pop(); close[ci].emplace((int)i, v); }
                 if ((int)far[ci].size() < P.m) far[ci].emplace((int)i, -v);
@@ -549,6 +576,17 @@ void oracle_stream_build(void* hv, const float* X, int64_t n) {
     auto* h = (OracleStream*)hv;
     Params& P = h->P;
     h->X.clear();
+    std::vector<float> Xc;
+    h->center.clear();
+    if (h->centering) {  // src/streamCEOs.cpp:22-34: rows minus vec_center
+        h->center.resize(P.d);
+        if (!h->given_center.empty()) h->center = h->given_center;
+        else column_mean_eigen(X, n, P.d, h->center.data());
+        Xc.resize((size_t)n * P.d);
+        for (int64_t i = 0; i < n; ++i)
+            for (int j = 0; j < P.d; ++j) Xc[(size_t)i * P.d + j] = X[(size_t)i * P.d + j] - h->center[j];
+        X = Xc.data();
+    }
     for (int64_t i = 0; i < n; ++i) h->X.emplace_back(X + (size_t)i * P.d, X + (size_t)(i + 1) * P.d);
     P.n = (int)n;
     make_bits(P);
@@ -566,6 +604,13 @@ int oracle_stream_update(void* hv, const float* Xn, int64_t n_new, int n_del) {
     const int n_cur = (int)h->X.size();
     if (n_del > n_cur) return 1;
     if (P.m > n_cur + n_new - n_del) return 1;
+    std::vector<float> Xc;
+    if (h->centering && !h->center.empty()) {  // src/streamCEOs.cpp:198-203
+        Xc.resize((size_t)n_new * P.d);
+        for (int64_t i = 0; i < n_new; ++i)
+            for (int j = 0; j < P.d; ++j) Xc[(size_t)i * P.d + j] = Xn[(size_t)i * P.d + j] - h->center[j];
+        Xn = Xc.data();
+    }
     for (int64_t i = 0; i < n_new; ++i) h->X.emplace_back(Xn + (size_t)i * P.d, Xn + (size_t)(i + 1) * P.d);
     for (int i = 0; i < n_del; ++i) h->X.pop_front();
     P.n = (int)h->X.size();
@@ -587,6 +632,7 @@ int oracle_stream_update(void* hv, const float* Xn, int64_t n_new, int n_del) {
         for (int64_t i = 0; i < n_new; ++i) {
             const float v = proj[(size_t)i * DR + c];
             const int id = (int)(i + n_cur);
+            if (std::isnan(v)) continue;  // NaN never enters a list (as build_buckets)
             if ((int)close.size() < P.m) close.emplace(id, v);
             else if (v > close.top().v) { close.pop(); close.emplace(id, v); }
             if ((int)far.size() < P.m) far.emplace(id, -v);
@@ -611,6 +657,21 @@ void oracle_stream_get_buckets(void* hv, int32_t* cid, float* cval, int32_t* fid
 }
 
 int oracle_stream_size(void* hv) { return (int)((OracleStream*)hv)->X.size(); }
+
+void oracle_column_mean(const float* X, int64_t n, int d, float* out) { column_mean_eigen(X, n, d, out); }
+
+// centering (src/streamCEOs.h:44); a given centre (d floats, or null to clear) replaces
+// the computed mean at the next build.
+void oracle_stream_set_centering(void* hv, int on, const float* given) {
+    auto* h = (OracleStream*)hv;
+    h->centering = on != 0;
+    h->given_center.clear();
+    if (given) h->given_center.assign(given, given + h->P.d);
+}
+void oracle_stream_get_center(void* hv, float* out) {
+    auto* h = (OracleStream*)hv;
+    std::memcpy(out, h->center.data(), h->center.size() * 4);
+}
 
 // src/streamCEOs.cpp:378-612 (estimate_search); guard s <= n_proj (src/streamCEOs.cpp:386).
 int oracle_stream_search(void* hv, const float* Q, int nq, int k, int s, int mp, int c,

@@ -67,6 +67,9 @@ def lib():
             "oracle_stream_update": (i32, [vp, _f32p, i64, i32]),
             "oracle_stream_get_buckets": (None, [vp, _i32p, _f32p, _i32p, _f32p, _i32p, _i32p]),
             "oracle_stream_size": (i32, [vp]),
+            "oracle_column_mean": (None, [_f32p, i64, i32, _f32p]),
+            "oracle_stream_set_centering": (None, [vp, i32, C.c_void_p]),
+            "oracle_stream_get_center": (None, [vp, _f32p]),
             "oracle_stream_search": (i32, [vp, _f32p, i32, i32, i32, i32, i32, _i32p, _f32p]),
             "oracle_ceos_search_hash": (i32, [vp, _f32p, i32, i32, i32, i32, _i32p, _f32p, _i32p]),
             "oracle_stream_hash_search": (i32, [vp, _f32p, i32, i32, i32, i32, _i32p, _f32p]),
@@ -122,6 +125,14 @@ def exact_mips(X, Q, k, threads=0):
     ips = np.zeros((nq, k), np.float32)
     lib().oracle_exact_mips(X, X.shape[0], X.shape[1], Q, nq, k, threads, ids, ips)
     return ids, ips
+
+
+def column_mean(X):
+    """FP32 column mean in the order of Eigen 3.4's colwise().mean() (src/streamCEOs.cpp:24-25)."""
+    X = _f32(X)
+    out = np.zeros(X.shape[1], np.float32)
+    lib().oracle_column_mean(X, X.shape[0], X.shape[1], out)
+    return out
 
 
 class OracleCEOs:
@@ -213,8 +224,18 @@ class OracleCEOs:
             return ids, ips, distinct, margin
         return ids, ips
 
-    def build_CEOs_Hash(self, X):
-        """CEOs-Hash keeps the coCEOs lists' ids (src/CEOs.cpp:709-902); no centering."""
+    centering = False  # src/CEOs.h:42
+    center = None
+
+    def build_CEOs_Hash(self, X, center=None):
+        """CEOs-Hash keeps the coCEOs lists' ids (src/CEOs.cpp:709-902). With centering the
+        stored points are X minus its column mean (or minus `center` when given); the
+        reference's own expression there (src/CEOs.cpp:721-728, a row-wise mean) only
+        type-checks when n equals d."""
+        X = _f32(X)
+        if self.centering:
+            self.center = column_mean(X) if center is None else np.asarray(center, np.float32)
+            X = np.ascontiguousarray(X - self.center[None, :], dtype=np.float32)
         self.build_coCEOs_Est(X)
 
     def search_CEOs_Hash(self, Q, n_neighbors, verbose=False, *, stats=False):
@@ -263,6 +284,17 @@ class OracleStream:
         self.n_probed_vectors = top_points  # src/streamCEOs.h:58
         lib().oracle_stream_set_index_param(self._h, n_proj, n_repeats, top_points, n_threads,
                                             random_seed)
+
+    def set_centering(self, on, center=None):
+        """centering (src/streamCEOs.h:44); `center` replaces the computed mean at the next build."""
+        c = None if center is None else np.ascontiguousarray(center, np.float32)
+        self._given = c  # kept alive while the library may read it
+        lib().oracle_stream_set_centering(self._h, 1 if on else 0, None if c is None else c.ctypes.data)
+
+    def center(self):
+        out = np.zeros(self.d, np.float32)
+        lib().oracle_stream_get_center(self._h, out)
+        return out
 
     def build(self, X):
         X = _f32(X)

@@ -330,3 +330,113 @@ def test_bucket_columns_equal_full_build(oracle):
     part = o.bucket_columns(X, cols)
     for a, b in zip(part, full):
         assert np.array_equal(a, b[cols])
+
+
+# ------------------------------------------------------------------ non-finite rows
+def test_nan_never_enters_a_list(oracle):
+    """Rows with a NaN project to NaN everywhere; they never enter a list (the strict '>'
+    of src/CEOs.cpp:395,406 never admits one; the heap-filling case is defined the same
+    way). A row with an infinite entry projects to +-Inf after the first transform and to
+    NaN after the second (Inf - Inf), so it is left out as well; infinite projections of
+    finite rows (overflow) would be ordinary values. Checked against a brute-force sort of
+    the non-NaN projections."""
+    n, d, D, R, m = 300, 16, 32, 2, 12
+    rng = np.random.default_rng(14)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    X[[0, 3, 150, 299], 5] = np.nan     # NaN rows, two of them while the heaps are filling
+    X[[7, 200], 2] = np.inf
+    X[[8], 9] = -np.inf
+    o = oracle.OracleCEOs(n, d)
+    o.setIndexParam(D, R, m, 2, 42)
+    o.build_coCEOs_Est(X)
+    P = o.project(X)
+    cid, cv, fid, fv = o.buckets()
+    ids = np.arange(n)
+    for c in range(D * R):
+        col = P[:, c]
+        ok = ~np.isnan(col)
+        order = ids[ok][np.lexsort((ids[ok], -col[ok].astype(np.float64)))][:m]
+        assert np.array_equal(cv[c], col[order]) and np.array_equal(cid[c], order), c
+        order = ids[ok][np.lexsort((ids[ok], col[ok].astype(np.float64)))][:m]
+        assert np.array_equal(fv[c], -col[order]) and np.array_equal(fid[c], order), c
+        assert not np.isin(cid[c], [0, 3, 7, 8, 150, 200, 299]).any()
+        assert not np.isin(fid[c], [0, 3, 7, 8, 150, 200, 299]).any()
+
+
+# ------------------------------------------------------------------ centering
+def test_column_mean_within_fp32_bound(oracle):
+    """The Eigen-order FP32 column mean is within the rounding bound of its summation
+    ((n/4 + 4) u sum|x| / n, u = 2^-24) of the FP64 mean, and exact on data whose sums
+    are exact in FP32."""
+    rng = np.random.default_rng(15)
+    for n, d in ((1, 3), (2, 5), (9, 7), (10, 7), (4097, 33), (200001, 16)):
+        X = (rng.standard_normal((n, d)) * 3 + 5).astype(np.float32)
+        c = oracle.column_mean(X)
+        m64 = X.astype(np.float64).mean(0)
+        bound = (n / 4 + 4) * 2.0 ** -24 * np.abs(X).astype(np.float64).mean(0) + 2.0 ** -24 * np.abs(m64)
+        assert np.all(np.abs(c - m64) <= bound), (n, d)
+    Xi = rng.integers(-1000, 1000, (1001, 6)).astype(np.float32)   # integer sums: exact
+    np.testing.assert_array_equal(oracle.column_mean(Xi), (Xi.astype(np.float64).sum(0) / 1001).astype(np.float32))
+
+
+def test_stream_centering_subtracts_the_build_mean(oracle):
+    """streamCEOs with centering stores rows minus the build rows' mean and subtracts the
+    same centre from later inserts (src/streamCEOs.cpp:22-34, 198-203): its lists equal
+    those of an uncentred index over the pre-centred rows."""
+    rng = np.random.default_rng(16)
+    X = (rng.standard_normal((3000, 24)) + 2.0).astype(np.float32)
+    Xn = (rng.standard_normal((500, 24)) + 2.0).astype(np.float32)
+    a = oracle.OracleStream(24)
+    a.setIndexParam(64, 2, 30, 2, 42)
+    a.set_centering(True)
+    a.build(X)
+    a.update(Xn, 200)
+    c = a.center()
+    np.testing.assert_array_equal(c, oracle.column_mean(X))
+    b = oracle.OracleStream(24)
+    b.setIndexParam(64, 2, 30, 2, 42)
+    b.build(X - c)
+    b.update(Xn - c, 200)
+    for x, y in zip(a.buckets(), b.buckets()):
+        assert np.array_equal(x, y)
+
+
+# ------------------------------------------------------------------ projection error
+def _proj64(X, bits, F, D, R):
+    """Sign + FWHT projections in FP64 (numpy, ascending stages), first D per repeat."""
+    n, d = X.shape
+    out = np.zeros((n, D * R))
+    for r in range(R):
+        y = np.zeros((n, F))
+        y[:, :d] = X
+        for i in range(3):
+            y *= 2 * bits[r * 3 * F + i * F: r * 3 * F + (i + 1) * F].astype(np.float64) - 1
+            h = 1
+            while h < F:
+                y = y.reshape(n, F // (2 * h), 2, h)
+                y = np.concatenate([y[:, :, :1] + y[:, :, 1:], y[:, :, :1] - y[:, :, 1:]], axis=2).reshape(n, F)
+                h *= 2
+        out[:, r * D:(r + 1) * D] = y[:, :D]
+    return out
+
+
+@pytest.mark.parametrize("F", [512, 1024, 4096])
+def test_projection_error_against_fp64(oracle, F):
+    """States the FP32 projection's error against FP64 (the device projection is bit-equal
+    to the oracle's, checked on the GPU): normwise relative error and the largest
+    coordinate error over the largest |coordinate|, on N(0,I) rows with d = F."""
+    rng = np.random.default_rng(17)
+    n, R = 64, 2
+    X = rng.standard_normal((n, F)).astype(np.float32)
+    o = oracle.OracleCEOs(n, F)
+    o.setIndexParam(F, R, 4, 2, 42)
+    assert o.F == F
+    P32 = o.project(X).astype(np.float64)
+    P64 = _proj64(X.astype(np.float64), oracle.sign_bits(F * 3 * R, 42), F, F, R)
+    normwise = np.linalg.norm(P32 - P64, axis=1) / np.linalg.norm(P64, axis=1)
+    coord = np.abs(P32 - P64).max(1) / np.abs(P64).max(1)
+    print(f"F={F}: normwise max {normwise.max():.3e} mean {normwise.mean():.3e}; "
+          f"max coordinate error / max |p| {coord.max():.3e}")
+    L = F.bit_length() - 1
+    assert normwise.max() <= 3 * L * 2.0 ** -24 * 4  # 3 rounds of L stages
+    assert coord.max() < 1e-5
