@@ -47,6 +47,89 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_info():
+    """nproc, the cores this process may run on, OMP_NUM_THREADS, CPU model and RAM."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["affinity"] = os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    info["omp_num_threads"] = int(omp) if omp.isdigit() else None
+    info["cpu_model"] = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal"):
+                    info["ram_gb"] = round(int(line.split()[1]) / 2 ** 20, 1)
+                    break
+    except OSError:
+        pass
+    return info
+
+
+def cpu_threads(args, info):
+    """Every core this process may use, capped by OMP_NUM_THREADS when set (the host's
+    share of a shared machine)."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    t = info.get("affinity") or 1
+    if info.get("omp_num_threads"):
+        t = min(t, info["omp_num_threads"])
+    return t
+
+
+def stream_update_baseline(O, timing, threads, n0, reps=3):
+    """streamCEOs.update of 10,000 new rows with n_delPoints = 10,000 at the streaming
+    configuration's parameters (d=384, D=512, R=8, m=1000), on an index of n0 points: the
+    reference's update (src/streamCEOs.cpp:174-369, as the oracle restates it) on the host
+    beside the device's, median of `reps` updates each. The cost of an update does not
+    grow with the index (new rows projected, each list merged with them)."""
+    import ctypes
+
+    from ceos_amd import capi
+    from ceos_amd.workloads import WORKLOADS, make_points
+
+    w = WORKLOADS["stream_5m"]
+    X0 = make_points(w, n0, seed=0)
+    news = [make_points(w, 10000, seed=40 + i) for i in range(reps)]
+    o = O.OracleStream(w.d, timing=timing)
+    o.setIndexParam(w.D, w.R, w.m, threads, w.seed)
+    o.build(X0)
+    cpu = []
+    for Xn in news:
+        t0 = time.perf_counter()
+        o.update(Xn, 10000)
+        cpu.append((time.perf_counter() - t0) * 1e3)
+    del o
+    L = capi.lib()
+    h = ctypes.c_void_p()
+    capi.check(L.ceos_stream_create(w.d, ctypes.byref(h)))
+    gpu = []
+    try:
+        capi.check(L.ceos_stream_set_index_param(h, w.D, w.R, w.m, -1, w.seed))
+        capi.check(L.ceos_stream_build(h, X0.ctypes.data, n0, w.d))
+        for Xn in news:
+            t0 = time.perf_counter()
+            capi.check(L.ceos_stream_update(h, Xn.ctypes.data, len(Xn), w.d, 10000))
+            gpu.append((time.perf_counter() - t0) * 1e3)
+    finally:
+        L.ceos_stream_destroy(h)
+    out = {"cpu_ms_per_10k": round(float(np.median(cpu)), 3), "gpu_ms_per_10k": round(float(np.median(gpu)), 3),
+           "sample": f"index of {n0} points x {w.d}, D={w.D}, R={w.R}, m={w.m}; +10,000 rows, n_delPoints 10,000; "
+                     f"median of {reps} (host: {threads} threads; device: host rows copied in)"}
+    log(f"stream update per 10K: cpu {out['cpu_ms_per_10k']} ms, gpu {out['gpu_ms_per_10k']} ms")
+    return out
+
+
 def exact_topk(torch, dX, dQ, k, chunk=256):
     out = []
     for a in range(0, dQ.shape[0], chunk):
@@ -87,8 +170,9 @@ def run_mode(args, w, X, Q, dX, dQ, n, d, nq, torch, CEOs):
     exact = exact_topk(torch, dX, dQ, k)
     rec = float(np.mean([len(np.intersect1d(exact[q], ids[q])) / k for q in range(nq)]))
     # CPU baseline: the oracle's search on a sample of the queries, same parameters
-    o = O.OracleCEOs(n, d)
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    o = O.OracleCEOs(n, d, timing=O.lib(timing=True) is not None)
+    info = host_info()
+    threads = cpu_threads(args, info)
     o.setIndexParam(w.D, w.R, w.m, threads, w.seed)
     (o.build if mode == "est" else o.build_CEOs_Hash)(X)
     o.n_probed_vectors, o.n_probed_points, o.n_cand = s, mp, c
@@ -128,7 +212,8 @@ def run_mode(args, w, X, Q, dX, dQ, n, d, nq, torch, CEOs):
                      "kernel": "k_est_search" if mode == "est" else "k_search_fused (hash mode)",
                      "bytes_per_query": round(per_q), "timed": "host entry point (PCIe copies of Q and results included)"},
         "cpu_baseline": {"value": round(cpu_qps, 1) if cpu_qps else None, "unit": "queries/s", "cores": threads,
-                         "kind": "port", "sample": f"{S} queries, {reps - 1} passes", "topk_match_vs_gpu": match},
+                         "kind": "port", "sample": f"{S} queries, {reps - 1} passes", "topk_match_vs_gpu": match,
+                         "host": info},
     }
     print(json.dumps(out), flush=True)
 
@@ -147,11 +232,18 @@ def main():
     ap.add_argument("--cpu-queries", type=int, default=0, help="CPU baseline sample (default: all queries)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="repeat the CPU sample for this long")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-build-points", type=int, default=50_000, help="CPU build baseline sample")
+    ap.add_argument("--cpu-build-points", type=int, default=100_000, help="CPU build baseline sample")
+    ap.add_argument("--stream-points", type=int, default=200_000,
+                    help="index size of the streamCEOs update baseline (host and device)")
+    ap.add_argument("--no-stream-baseline", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-qps", type=float, default=-1.0,
                     help="offered rate of the served run (default: 60%% of the batched rate; 0 = skip)")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-inflight", type=int, default=2, help="batch slots of the served run")
+    ap.add_argument("--serve-warmup-seconds", type=float, default=0.2,
+                    help="untimed served run at the same rate before the measured one")
+    ap.add_argument("--serve-max-batch", type=int, default=4096, help="queries per batch slot of the served run")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-rerank-bounds", action="store_true",
                     help="keep no int8 row copies: every candidate gets the exact FP32 rerank")
@@ -342,65 +434,107 @@ def main():
     served = None
     rate = args.serve_qps if args.serve_qps >= 0 else round(0.6 * qps, -3)
     if rate > 0:
-        nreq = max(1, int(rate * args.serve_seconds))
-        idx = np.arange(nreq) % nq
-        srv = CEOs.Server(index, max_batch=4096, k_max=k_max, s_max=s_max, c_max=c_max, inflight=2)
-        arrivals = np.cumsum(np.random.default_rng(3).exponential(1.0 / rate, nreq))
-        srv.submit(Q[idx], ks[idx], ss[idx], cs[idx], arrivals)
-        srv.drain()
+        srv = CEOs.Server(index, max_batch=args.serve_max_batch, k_max=k_max, s_max=s_max, c_max=c_max,
+                          inflight=args.serve_inflight)
+
+        def offer(seconds, seed):
+            n_r = max(1, int(rate * seconds))
+            idx = np.arange(n_r) % nq
+            arrivals = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, n_r))
+            # the request arrays stay alive until the run ends: freeing gigabytes while
+            # requests are being served stalls the process (the server copies them at submit)
+            reqs = (Q[idx], ks[idx], ss[idx], cs[idx])
+            srv.submit(*reqs, arrivals)
+            srv.drain()
+            del reqs
+            return n_r
+
+        offer(args.serve_warmup_seconds, 4)  # untimed: the server at the same load first
+        srv.reset_stats()
+        nreq = offer(args.serve_seconds, 3)
         st = srv.stats()
+        trace = srv.batches()
         srv.close()
+        slow = [{"launch_s": round(float(b[0]), 4), "enqueue_ms": round(float(b[1]), 3), "done_ms": round(float(b[2]), 3),
+                 "oldest_wait_ms": round(float(b[3]), 3), "queries": int(b[4])}
+                for b in trace[np.argsort(-trace[:, 2])[:3]]] if len(trace) else []
         served = {"offered_qps": rate, "achieved_qps": round(st["qps"], 1), "requests": nreq,
                   "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
                   "mean_batch": round(st["mean_batch"], 1), "inflight_slots": st["inflight_slots"],
+                  "wait_ms": {"p50": round(st["wait_p50_ms"], 3), "p99": round(st["wait_p99_ms"], 3),
+                              "max": round(st["wait_max_ms"], 3)},
+                  "batch_service_ms": {"p50": round(st["service_p50_ms"], 3), "p99": round(st["service_p99_ms"], 3),
+                                       "max": round(st["service_max_ms"], 3)},
+                  "slowest_batches": slow, "warmup_s": args.serve_warmup_seconds,
                   "vs_plain": round(st["qps"] / plain_qps, 1) if plain_qps else None}
         log(f"served: {st['qps']:,.0f} QPS offered {rate:,.0f}, p50 {st['p50_ms']:.3f} ms p99 {st['p99_ms']:.3f} ms, "
-            f"mean batch {st['mean_batch']:.1f}")
+            f"mean batch {st['mean_batch']:.1f}; slowest batches {slow}")
 
-    # ---- CPU baseline (oracle restatement of the reference search), same index
+    # ---- CPU baseline: the oracle's restatement of the reference's search and build,
+    # timed on this host's cores with the reference's FFHT, tsl::robin_map and an FP32
+    # SIMD dot (the timing build under oracle/_ref), on the same index and queries;
+    # median of 3 runs
     cpu = None
     c_eff = cs.astype(np.float64)
     if not args.no_cpu_baseline:
         from oracle import oracle as O
 
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        info = host_info()
+        threads = cpu_threads(args, info)
+        timing = O.lib(timing=True) is not None
         cid, cv, fid, fv = index.get_buckets()
-        o = O.OracleCEOs(n, d)
+        o = O.OracleCEOs(n, d, timing=timing)
         o.setIndexParam(w.D, w.R, w.m, threads, w.seed)
         o.load_index(X, cid, cv, fid, fv)
         o.n_probed_vectors, o.n_probed_points, o.n_cand = w.s, w.m, w.c
         S = min(args.cpu_queries or nq, nq)
         o.search_coCEOs_Est(Q[:64], 0, ks=ks[:64], ss=ss[:64], cs=cs[:64])  # warm caches / thread pool
-        t0 = time.perf_counter()
-        oi, op, distinct, _ = o.search_coCEOs_Est(Q[:S], 0, ks=ks[:S], ss=ss[:S], cs=cs[:S], stats=True)
-        reps = 1
-        while time.perf_counter() - t0 < args.cpu_seconds:
-            o.search_coCEOs_Est(Q[:S], 0, ks=ks[:S], ss=ss[:S], cs=cs[:S])
-            reps += 1
-        cpu_s = time.perf_counter() - t0
-        cpu_qps = S * reps / cpu_s
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oi, op, distinct, _ = o.search_coCEOs_Est(Q[:S], 0, ks=ks[:S], ss=ss[:S], cs=cs[:S], stats=True)
+            runs.append(time.perf_counter() - t0)
+        cpu_s = float(np.median(runs))
+        cpu_qps = S / cpu_s
         ooff = np.zeros(S + 1, np.int64)
         ooff[1:] = np.cumsum(ks[:S])
         match = float(np.mean([np.array_equal(oi[ooff[q]:ooff[q + 1]], ids[q, :ks[q]]) for q in range(S)]))
         c_eff[:S] = np.minimum(cs[:S], distinct)
-        log(f"cpu oracle: {cpu_qps:,.0f} QPS on {S} queries x {reps}, {threads} threads; top-k match {match:.4f}; "
-            f"mean distinct candidates {distinct.mean():.0f}")
-        # the oracle's build (the reference's: all projections, then a heap per list) on a
-        # bounded sample of the same points and parameters
+        log(f"cpu baseline: {cpu_qps:,.0f} QPS on {S} queries (median of 3: {[round(r, 2) for r in runs]} s), "
+            f"{threads} threads; top-k match {match:.4f}; mean distinct candidates {distinct.mean():.0f}")
+        # one query per call on one thread: the per-query latency of the reference's search
+        o.set_threads(1)
+        lat1 = []
+        for q in range(min(200, nq)):
+            t0 = time.perf_counter()
+            o.search_coCEOs_Est(Q[q:q + 1], 0, ks=ks[q:q + 1], ss=ss[q:q + 1], cs=cs[q:q + 1])
+            lat1.append((time.perf_counter() - t0) * 1e3)
+        del o
+        # the build (the reference's: all projections, then a heap per list) on a bounded
+        # sample of the same points and parameters
         nb = min(n, max(w.m, args.cpu_build_points))
-        ob = O.OracleCEOs(nb, d)
-        ob.setIndexParam(w.D, w.R, w.m, threads, w.seed)
-        t0 = time.perf_counter()
-        ob.build_coCEOs_Est(X[:nb])
-        cpu_build_s = time.perf_counter() - t0
-        del ob
-        cpu_build = {"mpts_s": round(nb / cpu_build_s / 1e6, 4), "sample": f"first {nb} points", "cores": threads,
-                     "seconds": round(cpu_build_s, 2)}
-        log(f"cpu oracle build: {nb / cpu_build_s / 1e6:.4f} Mpts/s on {nb} points, {threads} threads")
+        bt = []
+        for _ in range(3):
+            ob = O.OracleCEOs(nb, d, timing=timing)
+            ob.setIndexParam(w.D, w.R, w.m, threads, w.seed)
+            t0 = time.perf_counter()
+            ob.build_coCEOs_Est(X[:nb])
+            bt.append(time.perf_counter() - t0)
+            del ob
+        cpu_build_s = float(np.median(bt))
+        cpu_build = {"mpts_s": round(nb / cpu_build_s / 1e6, 4), "sample": f"first {nb} points, median of 3",
+                     "cores": threads, "seconds": round(cpu_build_s, 3)}
+        log(f"cpu baseline build: {nb / cpu_build_s / 1e6:.4f} Mpts/s on {nb} points, {threads} threads")
+        stream_upd = None if args.no_stream_baseline else stream_update_baseline(O, timing, threads, args.stream_points)
+        what = ("the oracle restatement compiled with the reference's FFHT (src/fht.c), tsl::robin_map (src/tsl) "
+                "and an FP32 AVX2 dot" if timing else "the oracle restatement (FP64 dot, textbook FWHT)")
         cpu = {"value": round(cpu_qps, 1), "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"{S} of the {nq} queries, {reps} passes, on the same index (buckets from the device "
-                         f"build, verified identical to the oracle's build in tests); {cpu_s:.1f} s",
-               "topk_match_vs_gpu": match, "build": cpu_build}
+               "sample": f"{S} of the {nq} queries on the same index (buckets from the device build, identical to "
+                         f"the oracle's build in tests), median of 3 runs ({cpu_s:.2f} s); {what}",
+               "topk_match_vs_gpu": match,
+               "latency_ms_1thread": {"p50": round(float(np.percentile(lat1, 50)), 3),
+                                      "p99": round(float(np.percentile(lat1, 99)), 3), "queries": len(lat1)},
+               "build": cpu_build, "stream_update": stream_upd, "host": info}
 
     # ---- roofline of the fused search kernel. Algorithmic bytes per query: the query row,
     # the probed bucket entries (8 B each), the candidates' rows and the top-k output;

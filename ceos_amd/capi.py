@@ -57,6 +57,14 @@ def lib():
             "ceos_set_kernel_timing": (i32, [vp, i32]),
             "ceos_get_kernel_timing": (i32, [vp, P, P, P]),
             "ceos_read_bandwidth": (i32, [P, i64, P, i64, i32, vp, P]),
+            "ceos_stream_create": (i32, [i32, C.POINTER(C.c_void_p)]),
+            "ceos_stream_destroy": (None, [vp]),
+            "ceos_stream_set_index_param": (i32, [vp, i32, i32, i32, i32, i32]),
+            "ceos_stream_build": (i32, [vp, P, i64, i32]),
+            "ceos_stream_update": (i32, [vp, P, i64, i32, i32]),
+            "ceos_stream_get_build_times": (i32, [vp, P]),
+            "ceos_get_build_times": (i32, [vp, P]),
+            "ceos_get_search_times": (i32, [vp, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

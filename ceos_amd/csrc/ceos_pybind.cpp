@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/ceos_amd.h"
 
@@ -666,7 +667,33 @@ class PyServer {
         o["insert_max_ms"] = st.insert_max_ms;
         o["delete_mean_ms"] = st.delete_mean_ms;
         o["delete_max_ms"] = st.delete_max_ms;
+        o["wait_p50_ms"] = st.wait_p50_ms;
+        o["wait_p99_ms"] = st.wait_p99_ms;
+        o["wait_max_ms"] = st.wait_max_ms;
+        o["service_p50_ms"] = st.service_p50_ms;
+        o["service_p99_ms"] = st.service_p99_ms;
+        o["service_max_ms"] = st.service_max_ms;
         return o;
+    }
+    // Per-batch trace: columns launch_s, enqueue_ms, done_ms, oldest_wait_ms, queries, slot.
+    py::array_t<double> batches() {
+        int64_t n = 0;
+        check(ceos_server_get_batches(live(), nullptr, 0, &n));
+        std::vector<ceos_server_batch> b((size_t)n);
+        int64_t got = 0;
+        check(ceos_server_get_batches(live(), b.data(), n, &got));
+        n = std::min(n, got);
+        py::array_t<double> out({(py::ssize_t)n, (py::ssize_t)6});
+        auto m = out.mutable_unchecked<2>();
+        for (int64_t i = 0; i < n; ++i) {
+            m(i, 0) = b[i].launch_s;
+            m(i, 1) = b[i].enqueue_ms;
+            m(i, 2) = b[i].done_ms;
+            m(i, 3) = b[i].oldest_wait_ms;
+            m(i, 4) = b[i].queries;
+            m(i, 5) = b[i].slot;
+        }
+        return out;
     }
     void reset_stats() { check(ceos_server_reset_stats(live())); }
 
@@ -743,6 +770,7 @@ PYBIND11_MODULE(CEOs, m) {
         .def("delete_ids", &PyServer::remove, py::arg("ids"))
         .def("drain", &PyServer::drain)
         .def("stats", &PyServer::stats)
+        .def("batches", &PyServer::batches)
         .def("reset_stats", &PyServer::reset_stats)
         .def("close", &PyServer::close);
 

@@ -86,6 +86,8 @@ struct Slot {
     HBuf<int32_t> hk, hs, hc, hI;
     std::vector<int64_t> tickets;
     bool busy = false;
+    Clock::time_point t_launch{};
+    ceos_server_batch cur{};  // trace record of the batch in flight
 };
 
 }  // namespace
@@ -116,6 +118,10 @@ struct ceos_server {
     // statistics
     int64_t submitted = 0, completed = 0, batches = 0, batched_queries = 0;
     std::vector<double> lat_ms;
+    std::vector<double> wait_ms;     // per request: arrival -> its batch launched
+    std::vector<double> service_ms;  // per batch: launch -> results in host memory
+    std::vector<ceos_server_batch> trace;  // per batch (ceos_server_get_batches), at most kMaxTrace
+    static constexpr size_t kMaxTrace = 1 << 20;
     int64_t n_upd[2] = {0, 0};                           // inserts, deletes applied
     double upd_ms_sum[2] = {0.0, 0.0}, upd_ms_max[2] = {0.0, 0.0};
     Clock::time_point first_arrival{}, last_done{};
@@ -154,22 +160,23 @@ struct ceos_server {
         Feed* f = due.top().second;
         due.pop();
         const int i = f->order[f->next++];
-        if (f->pending())
-            due.push({f->next_arrival(), f});
-        else {  // every row admitted: drop the rows
-            f->Q.clear();
-            f->Q.shrink_to_fit();
-        }
+        if (f->pending()) due.push({f->next_arrival(), f});
         --queued;
         return i;
     }
-    void release_if_taken(Feed* f) {
-        if (f->untaken > 0 || f->pending()) return;
+    // Removes a submission whose results were all collected and returns it, to be freed
+    // by the caller after releasing the lock (its rows can be gigabytes: freeing them
+    // under the lock would stall the worker).
+    std::unique_ptr<Feed> release_if_taken(Feed* f) {
+        if (f->untaken > 0 || f->pending()) return nullptr;
         int idx = 0;
-        if (find(f->first_ticket, &idx) != f) return;
+        if (find(f->first_ticket, &idx) != f) return nullptr;
         auto it = std::upper_bound(feeds.begin(), feeds.end(), f->first_ticket,
                                    [](int64_t t, const std::unique_ptr<Feed>& g) { return t < g->first_ticket; });
-        feeds.erase(--it);
+        --it;
+        std::unique_ptr<Feed> out = std::move(*it);
+        feeds.erase(it);
+        return out;
     }
 
     // Moves a finished slot's results to its requests (mu held).
@@ -190,6 +197,9 @@ struct ceos_server {
             ++completed;
         }
         last_done = now;
+        service_ms.push_back(std::chrono::duration<double, std::milli>(now - sl.t_launch).count());
+        sl.cur.done_ms = service_ms.back();
+        if (trace.size() < kMaxTrace) trace.push_back(sl.cur);
         sl.tickets.clear();
         sl.busy = false;
         cv_done.notify_all();
@@ -209,27 +219,62 @@ struct ceos_server {
 
     // Launches one batch of arrived requests into a free slot (mu held, released
     // around the device calls).
+    struct Taken {
+        Feed* f;
+        int i;
+        float cost;
+    };
+    std::vector<Taken> taken_;  // the batch being launched (worker thread only)
+
     void launch(Slot& sl, std::unique_lock<std::mutex>& lk) {
         const auto now = Clock::now();
         const int d = core->d;
-        int nb = 0;
-        while (nb < cfg.max_batch) {
+        const float mp = (float)std::min(core->mp, core->m);
+        taken_.clear();
+        while ((int)taken_.size() < cfg.max_batch) {
             Feed* f = earliest();
             if (!f || f->next_arrival() > now) break;
             const int i = f->order[f->next];
-            std::memcpy(sl.hQ.p + (size_t)nb * d, f->Q.data() + (size_t)i * d, (size_t)d * 4);
-            sl.hk.p[nb] = f->k[i];
-            sl.hs.p[nb] = f->s[i];
-            sl.hc.p[nb] = f->c[i];
-            sl.tickets.push_back(f->first_ticket + i);
-            ++nb;
             admit();
+            wait_ms.push_back(std::chrono::duration<double, std::milli>(now - f->arrival(i)).count());
+            // bytes the search reads for this request: its probed bucket entries, its
+            // candidates' int8 rows (or FP32 rows), its k exact rows
+            const float cost = 8.0f * (float)f->s[i] * mp + (float)(d + 16) * (float)f->c[i] + 4.0f * d * (float)f->k[i];
+            taken_.push_back({f, i, cost});
         }
-        if (nb == 0) return;
+        if (taken_.empty()) return;
+        // Largest first: the persistent grid hands the batch's queries out in order, so the
+        // heavy ones start first and light ones fill the workgroups that finish early; a
+        // batch lasts as long as its slowest workgroup.
+        std::stable_sort(taken_.begin(), taken_.end(), [](const Taken& a, const Taken& b) { return a.cost > b.cost; });
+        const int nb = (int)taken_.size();
+        for (int b = 0; b < nb; ++b) {
+            const Feed* f = taken_[b].f;
+            const int i = taken_[b].i;
+            sl.hk.p[b] = f->k[i];
+            sl.hs.p[b] = f->s[i];
+            sl.hc.p[b] = f->c[i];
+            sl.tickets.push_back(f->first_ticket + i);
+        }
         sl.busy = true;
+        sl.t_launch = now;
+        sl.cur = ceos_server_batch{};
+        sl.cur.launch_s = std::chrono::duration<double>(now - first_arrival).count();
+        sl.cur.oldest_wait_ms = 0.0;
+        for (const Taken& t : taken_)
+            sl.cur.oldest_wait_ms =
+                std::max(sl.cur.oldest_wait_ms, std::chrono::duration<double, std::milli>(now - t.f->arrival(t.i)).count());
+        sl.cur.queries = nb;
+        for (size_t j = 0; j < slots.size(); ++j)
+            if (slots[j].get() == &sl) sl.cur.slot = (int32_t)j;
         ++batches;
         batched_queries += nb;
         lk.unlock();
+        // rows to the pinned staging outside the lock; a submission keeps its rows until
+        // all its results are collected (it is freed then, outside the lock), and no result
+        // of this batch can be collected before it completes, so the submissions stay alive
+        for (int b = 0; b < nb; ++b)
+            std::memcpy(sl.hQ.p + (size_t)b * d, taken_[b].f->Q.data() + (size_t)taken_[b].i * d, (size_t)d * 4);
         try {
             float* dQ;
             int32_t *dk, *ds, *dc, *dI;
@@ -251,6 +296,7 @@ struct ceos_server {
             return;
         }
         lk.lock();
+        sl.cur.enqueue_ms = std::chrono::duration<double, std::milli>(Clock::now() - sl.t_launch).count();
     }
 
     void apply_update(Update& u, std::unique_lock<std::mutex>& lk) {
@@ -395,6 +441,32 @@ int make_server(Core& x, ceos_index* hi, ceos_stream* hs, const ceos_server_conf
             sl->hc.alloc(cfg->max_batch);
             sl->hI.alloc((size_t)cfg->max_batch * cfg->k_max);
             sl->hP.alloc((size_t)cfg->max_batch * cfg->k_max);
+            {
+                // one real batch through the path the worker takes (staging copies, the
+                // graph on a non-empty batch, result copies), so that the first requests do
+                // not pay first-use costs: a slow first batch lets requests pile up into
+                // larger, slower batches at the start
+                const int nb = std::min(cfg->max_batch, 64);
+                for (int i = 0; i < nb; ++i) {
+                    for (int j = 0; j < x.d; ++j)
+                        sl->hQ.p[(size_t)i * x.d + j] = (float)(((i * 131 + j * 71) % 17) - 8) / 8.0f;
+                    sl->hk.p[i] = std::min(cfg->k_max, 10);
+                    sl->hs.p[i] = std::min(cfg->s_max, 16);
+                    sl->hc.p[i] = std::min(cfg->c_max, 500);
+                }
+                float* dQ;
+                int32_t *dk, *ds, *dc, *dI;
+                float* dP;
+                if (ceos_plan_buffers(sl->plan, &dQ, &dk, &ds, &dc, &dI, &dP) != CEOS_OK) throw DevError(g_err);
+                HIP_CHECK(hipMemcpyAsync(dQ, sl->hQ.p, (size_t)nb * x.d * 4, hipMemcpyHostToDevice, sl->st));
+                HIP_CHECK(hipMemcpyAsync(dk, sl->hk.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl->st));
+                HIP_CHECK(hipMemcpyAsync(ds, sl->hs.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl->st));
+                HIP_CHECK(hipMemcpyAsync(dc, sl->hc.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl->st));
+                if (ceos_plan_run(sl->plan, nb, sl->st) != CEOS_OK) throw DevError(g_err);
+                HIP_CHECK(hipMemcpyAsync(sl->hI.p, dI, (size_t)nb * cfg->k_max * 4, hipMemcpyDeviceToHost, sl->st));
+                HIP_CHECK(hipMemcpyAsync(sl->hP.p, dP, (size_t)nb * cfg->k_max * 4, hipMemcpyDeviceToHost, sl->st));
+                HIP_CHECK(hipStreamSynchronize(sl->st));
+            }
             srv->slots.push_back(std::move(sl));
         }
         srv->device_bytes = per * want;
@@ -515,9 +587,12 @@ int ceos_server_result(ceos_server* s, int64_t ticket, int wait, int32_t* ids, f
     f->ids[idx] = std::vector<int32_t>();
     f->ips[idx] = std::vector<float>();
     --f->untaken;
-    s->release_if_taken(f);
+    std::unique_ptr<Feed> gone = s->release_if_taken(f);
+    const std::string err = failed ? s->worker_error : std::string();
+    lk.unlock();
+    gone.reset();  // a finished submission is freed outside the lock
     if (failed) {
-        g_err = s->worker_error;
+        g_err = err;
         return CEOS_EDEVICE;
     }
     g_err.clear();
@@ -597,6 +672,16 @@ int ceos_server_get_stats(ceos_server* s, ceos_server_stats* out) {
             st.elapsed_s = std::chrono::duration<double>(s->last_done - s->first_arrival).count();
             st.qps = st.elapsed_s > 0 ? (double)s->completed / st.elapsed_s : 0.0;
         }
+        auto pcts = [](std::vector<double> x, double& p50, double& p99, double& mx) {
+            if (x.empty()) return;
+            std::sort(x.begin(), x.end());
+            auto at = [&](double p) { return x[std::min(x.size() - 1, (size_t)(p * (double)(x.size() - 1) + 0.5))]; };
+            p50 = at(0.50);
+            p99 = at(0.99);
+            mx = x.back();
+        };
+        pcts(s->wait_ms, st.wait_p50_ms, st.wait_p99_ms, st.wait_max_ms);
+        pcts(s->service_ms, st.service_p50_ms, st.service_p99_ms, st.service_max_ms);
         st.inflight_slots = (int)s->slots.size();
         st.device_bytes = s->device_bytes;
         st.inserts = s->n_upd[0];
@@ -609,12 +694,25 @@ int ceos_server_get_stats(ceos_server* s, ceos_server_stats* out) {
     });
 }
 
+int ceos_server_get_batches(ceos_server* s, ceos_server_batch* out, int64_t max, int64_t* count) {
+    return guarded([&] {
+        if (!s || !count || (max > 0 && !out)) throw ArgError("null argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        *count = (int64_t)s->trace.size();
+        const int64_t n = std::min<int64_t>(std::max<int64_t>(max, 0), (int64_t)s->trace.size());
+        if (n > 0) std::memcpy(out, s->trace.data(), (size_t)n * sizeof(ceos_server_batch));
+    });
+}
+
 int ceos_server_reset_stats(ceos_server* s) {
     return guarded([&] {
         if (!s) throw ArgError("null server");
         std::lock_guard<std::mutex> g(s->mu);
         s->submitted = s->completed = s->batches = s->batched_queries = 0;
         s->lat_ms.clear();
+        s->wait_ms.clear();
+        s->service_ms.clear();
+        s->trace.clear();
         s->any_arrival = false;
         s->n_upd[0] = s->n_upd[1] = 0;
         s->upd_ms_sum[0] = s->upd_ms_sum[1] = s->upd_ms_max[0] = s->upd_ms_max[1] = 0.0;

@@ -261,6 +261,10 @@ typedef struct {
     int64_t device_bytes;
     int64_t inserts, deletes;  /* streamCEOs updates applied */
     double insert_mean_ms, insert_max_ms, delete_mean_ms, delete_max_ms;  /* time to apply one */
+    /* where latency goes: queue wait (arrival -> its batch launched) per request, and batch
+     * service (launch -> results in host memory) per batch */
+    double wait_p50_ms, wait_p99_ms, wait_max_ms;
+    double service_p50_ms, service_p99_ms, service_max_ms;
 } ceos_server_stats;
 
 int ceos_server_create(ceos_index* h, const ceos_server_config* cfg, ceos_server** out);
@@ -283,6 +287,18 @@ int ceos_server_delete(ceos_server* s, const int32_t* ids, int64_t count);
 /* Waits until nothing is queued or running; reports a failed update. */
 int ceos_server_drain(ceos_server* s);
 int ceos_server_get_stats(ceos_server* s, ceos_server_stats* out);
+/* Per-batch trace since the last reset (up to 2^20 batches): when each batch was launched,
+ * how long the worker took to enqueue it, when its results were seen in host memory, the
+ * wait of its oldest request, its size and its slot. count gets the number recorded;
+ * at most max are copied. */
+typedef struct {
+    double launch_s;        /* seconds after the first arrival */
+    double enqueue_ms;      /* launch -> rows copied and device work enqueued */
+    double done_ms;         /* launch -> results in host memory, when the worker reaped them */
+    double oldest_wait_ms;  /* arrival of the batch's oldest request -> launch */
+    int32_t queries, slot;
+} ceos_server_batch;
+int ceos_server_get_batches(ceos_server* s, ceos_server_batch* out, int64_t max, int64_t* count);
 int ceos_server_reset_stats(ceos_server* s);
 
 /* --------------------------- class streamCEOs --------------------------- */

@@ -36,6 +36,19 @@
 
 #include <omp.h>
 
+// Timing build (oracle/_ref/libceos_ref_timing.so, made by the Makefile only where the
+// reference tree exists): the same restatement with the reference's own FFHT (src/fht.c,
+// bit-identical to fwht below), the reference's vendored tsl::robin_map for the score map
+// (src/tsl, as src/CEOs.cpp:558) and an FP32 AVX2 dot product in place of the FP64 one
+// (as Eigen's SIMD dot, src/CEOs.cpp:646). It is the CPU baseline the benchmark times;
+// parity checks use the default build.
+#ifdef CEOS_REF_TIMING
+#include <immintrin.h>
+
+#include "fht.h"
+#include "tsl/robin_map.h"
+#endif
+
 namespace {
 
 struct IFPair {  // (id, value) ordered by value only: src/Header.h:53-80
@@ -94,7 +107,11 @@ void project_row(const Params& P, const float* x, float* out /* D*R */) {
         for (int i = 0; i < 3; ++i) {
             for (int j = 0; j < P.F; ++j)
                 y[j] *= (2 * (float)P.bits[b0 + (size_t)i * P.F + j] - 1);
+#ifdef CEOS_REF_TIMING
+            fht_float(y.data(), L);
+#else
             fwht(y.data(), L);
+#endif
         }
         std::memcpy(out + (size_t)P.D * r, y.data(), sizeof(float) * P.D);
     }
@@ -170,6 +187,25 @@ std::vector<int> probe_order(const float* p, int DR, int s) {
     return order;
 }
 
+#ifdef CEOS_REF_TIMING
+float dot_f32(const float* a, const float* b, int d) {
+    __m256 s0 = _mm256_setzero_ps(), s1 = _mm256_setzero_ps();
+    int j = 0;
+    for (; j + 16 <= d; j += 16) {
+        s0 = _mm256_fmadd_ps(_mm256_loadu_ps(a + j), _mm256_loadu_ps(b + j), s0);
+        s1 = _mm256_fmadd_ps(_mm256_loadu_ps(a + j + 8), _mm256_loadu_ps(b + j + 8), s1);
+    }
+    for (; j + 8 <= d; j += 8) s0 = _mm256_fmadd_ps(_mm256_loadu_ps(a + j), _mm256_loadu_ps(b + j), s0);
+    s0 = _mm256_add_ps(s0, s1);
+    __m128 h = _mm_add_ps(_mm256_castps256_ps128(s0), _mm256_extractf128_ps(s0, 1));
+    h = _mm_hadd_ps(h, h);
+    h = _mm_hadd_ps(h, h);
+    float r = _mm_cvtss_f32(h);
+    for (; j < d; ++j) r += a[j] * b[j];
+    return r;
+}
+#endif
+
 double dot64(const float* a, const float* b, int d) {
     double acc = 0.0;
     for (int j = 0; j < d; ++j) acc += (double)a[j] * (double)b[j];
@@ -182,7 +218,11 @@ void rerank(const float* q, int d, const std::vector<int>& cands_asc,
             const std::function<const float*(int)>& row, int k, int32_t* out_id, float* out_ip) {
     MinHeap top;
     for (int id : cands_asc) {
+#ifdef CEOS_REF_TIMING
+        const float ip = dot_f32(q, row(id), d);
+#else
         const float ip = (float)dot64(q, row(id), d);
+#endif
         if ((int)top.size() < k) top.emplace(id, ip);
         else if (ip > top.top().v) { top.pop(); top.emplace(id, ip); }
     }
@@ -204,7 +244,11 @@ void search_one(const Params& P, const Buckets& B, const float* q, int k, int s,
     project_row(P, q, proj.data());
     std::vector<int> probes = probe_order(proj.data(), DR, s);
 
+#ifdef CEOS_REF_TIMING
+    tsl::robin_map<int, int> slot;
+#else
     std::unordered_map<int, int> slot;
+#endif
     slot.reserve((size_t)s * mp * 2);
     std::vector<int> ids;
     std::vector<float> sums;
@@ -341,6 +385,10 @@ void oracle_ceos_set_index_param(void* hv, int D, int R, int m, int threads, int
     if (seed >= 0) make_bits(h->P);  // deterministic: the same bits build() draws
 }
 int oracle_ceos_fht_dim(void* hv) { return ((OracleCEOs*)hv)->P.F; }
+// set_threads (src/CEOs.h:96-102): t <= 0 = every thread OpenMP offers.
+void oracle_ceos_set_threads(void* hv, int threads) {
+    ((OracleCEOs*)hv)->P.threads = threads > 0 ? threads : omp_get_max_threads();
+}
 
 
 // src/CEOs.cpp:311-441 (build_coCEOs_Est)
@@ -419,7 +467,11 @@ void oracle_ceos_bucket_columns(void* hv, const float* X, int64_t n, const int* 
                 const size_t b0 = (size_t)P.F * 3 * reps[ri];
                 for (int rd = 0; rd < 3; ++rd) {
                     for (int j = 0; j < P.F; ++j) y[j] *= (2 * (float)P.bits[b0 + (size_t)rd * P.F + j] - 1);
+#ifdef CEOS_REF_TIMING
+                    fht_float(y.data(), L);
+#else
                     fwht(y.data(), L);
+#endif
                 }
                 std::memcpy(&proj[((size_t)(i - a) * reps.size() + ri) * P.D], y.data(), sizeof(float) * P.D);
             }

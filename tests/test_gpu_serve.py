@@ -136,3 +136,27 @@ def test_stream_server_updates_between_batches(CEOs):
         srv.drain()
         assert st.size() == 4000 + (step + 1) * 300
     srv.close()
+
+
+def test_serve_batch_trace(CEOs, index):
+    """The per-batch trace covers every served request, in launch order, with the wait of
+    each batch's oldest request and its service time."""
+    X, g = index
+    n = 400
+    Q = _rand(n, 64, 61)
+    k = np.full(n, 10, np.int32)
+    s = np.full(n, 20, np.int32)
+    c = np.full(n, 500, np.int32)
+    srv = CEOs.Server(g, max_batch=64, k_max=10, s_max=20, c_max=500, inflight=2)
+    srv.submit(Q, k, s, c, np.cumsum(np.random.default_rng(62).exponential(1.0 / 50000.0, n)))
+    srv.drain()
+    B = srv.batches()
+    st = srv.stats()
+    assert B.shape == (st["batches"], 6)
+    assert int(B[:, 4].sum()) == n and (B[:, 4] <= 64).all()
+    assert np.all(np.diff(B[:, 0]) >= 0)             # launch order
+    assert np.all(B[:, 2] >= B[:, 1]) and np.all(B[:, 3] >= 0)
+    assert set(B[:, 5].astype(int).tolist()) <= {0, 1}
+    srv.reset_stats()
+    assert srv.batches().shape == (0, 6)
+    srv.close()

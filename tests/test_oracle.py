@@ -440,3 +440,24 @@ def test_projection_error_against_fp64(oracle, F):
     L = F.bit_length() - 1
     assert normwise.max() <= 3 * L * 2.0 ** -24 * 4  # 3 rounds of L stages
     assert coord.max() < 1e-5
+
+
+def test_timing_build_computes_what_the_oracle_computes(oracle):
+    """The timed CPU baseline (reference FFHT, tsl::robin_map, FP32 SIMD dot) builds the
+    same lists and returns the same top-k as the parity oracle."""
+    if oracle.lib(timing=True) is None:
+        pytest.skip("timing build absent (the reference tree was not present at build time)")
+    rng = np.random.default_rng(18)
+    X = (rng.standard_normal((5000, 64)) * rng.lognormal(0, 0.5, (5000, 1))).astype(np.float32)
+    Q = rng.standard_normal((100, 64)).astype(np.float32)
+    out = []
+    for timing in (False, True):
+        o = oracle.OracleCEOs(*X.shape, timing=timing)
+        o.setIndexParam(256, 2, 60, 2, 42)
+        o.build_coCEOs_Est(X)
+        o.n_probed_vectors, o.n_cand = 20, 300
+        out.append((o.buckets(), o.search_coCEOs_Est(Q, 10)))
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][1][0], out[1][1][0])
+    np.testing.assert_allclose(out[0][1][1], out[1][1][1], rtol=1e-5, atol=1e-6)
