@@ -287,7 +287,8 @@ int ceos_server_delete(ceos_server* s, const int32_t* ids, int64_t count);
 /* Waits until nothing is queued or running; reports a failed update. */
 int ceos_server_drain(ceos_server* s);
 int ceos_server_get_stats(ceos_server* s, ceos_server_stats* out);
-/* Per-batch trace since the last reset (up to 2^20 batches): when each batch was launched,
+/* Per-batch trace since the last reset (up to 2^20 batches, in the order they completed):
+ * when each batch was launched,
  * how long the worker took to enqueue it, when its results were seen in host memory, the
  * wait of its oldest request, its size and its slot. count gets the number recorded;
  * at most max are copied. */

@@ -28,15 +28,3 @@ def CEOs():
 
     return mod
 
-
-@pytest.fixture(scope="session", autouse=True)
-def _torch_device_first(request):
-    """torch ships its own HIP runtime beside the one libceos_amd links: when both are used
-    in one process, torch's device is initialised first (as bench.py does), before any
-    device work of the library."""
-    if any(item.get_closest_marker("gpu") for item in request.session.items):
-        import torch
-
-        if torch.cuda.is_available():
-            torch.cuda.init()
-    yield

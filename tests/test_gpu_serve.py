@@ -139,8 +139,8 @@ def test_stream_server_updates_between_batches(CEOs):
 
 
 def test_serve_batch_trace(CEOs, index):
-    """The per-batch trace covers every served request, in launch order, with the wait of
-    each batch's oldest request and its service time."""
+    """The per-batch trace (in the order batches completed) covers every served request,
+    with the wait of each batch's oldest request and its service time."""
     X, g = index
     n = 400
     Q = _rand(n, 64, 61)
@@ -154,7 +154,7 @@ def test_serve_batch_trace(CEOs, index):
     st = srv.stats()
     assert B.shape == (st["batches"], 6)
     assert int(B[:, 4].sum()) == n and (B[:, 4] <= 64).all()
-    assert np.all(np.diff(B[:, 0]) >= 0)             # launch order
+    assert np.all(B[:, 0] >= 0) and B[:, 0].max() < st["elapsed_s"] + 1.0   # launched within the run
     assert np.all(B[:, 2] >= B[:, 1]) and np.all(B[:, 3] >= 0)
     assert set(B[:, 5].astype(int).tolist()) <= {0, 1}
     srv.reset_stats()

@@ -47,43 +47,42 @@ def test_list_searches_refused_on_a_ceos_est_index(CEOs):
 
 
 def test_device_searches_on_two_streams_match_one_at_a_time():
-    import torch
-
     from ceos_amd import capi
+    from tests._hip import DeviceArray, Stream, synchronize
 
-    dev = torch.device("cuda:0")
     n, d, nq, k = 40000, 64, 600, 10
     X = _rand(n, d, 3)
     Q = _rand(nq, d, 4)
     idx = capi.DeviceIndex(n, d)
     idx.set_index_param(256, 4, 200, -1, 42)
-    dX = torch.from_numpy(X).to(dev)
-    dQ = torch.from_numpy(Q).to(dev)
-    idx.build_device(dX.data_ptr(), n)
+    dX = DeviceArray(X)
+    dQ = DeviceArray(Q)
+    idx.build_device(dX.ptr, n)
     idx.set_query_param(40, 200, 1000)
 
     def run(stream, q0, q1, out_i, out_p):
-        idx.search_device(dQ[q0].data_ptr(), q1 - q0, k, out_i[q0].data_ptr(), out_p[q0].data_ptr(), stream)
+        idx.search_device(dQ.at(q0), q1 - q0, k, out_i.at(q0), out_p.at(q0), stream)
 
     # one at a time, synchronised
-    ri = torch.zeros((nq, k), dtype=torch.int32, device=dev)
-    rp = torch.zeros((nq, k), dtype=torch.float32, device=dev)
+    ri = DeviceArray(shape=(nq, k), dtype=np.int32)
+    rp = DeviceArray(shape=(nq, k), dtype=np.float32)
     for q0 in range(0, nq, 100):
         run(0, q0, q0 + 100, ri, rp)
-        torch.cuda.synchronize()
-    ref_i, ref_p = ri.cpu().numpy(), rp.cpu().numpy()
+        synchronize()
+    ref_i, ref_p = ri.numpy(), rp.numpy()
     # alternating between two caller streams and the index's own, no synchronisation
-    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    sa, sb = Stream(), Stream()
     for rep in range(3):
-        gi = torch.zeros((nq, k), dtype=torch.int32, device=dev)
-        gp = torch.zeros((nq, k), dtype=torch.float32, device=dev)
-        torch.cuda.synchronize()
+        gi = DeviceArray(shape=(nq, k), dtype=np.int32)
+        gp = DeviceArray(shape=(nq, k), dtype=np.float32)
+        gi.fill(0)
+        gp.fill(0)
         for j, q0 in enumerate(range(0, nq, 100)):
-            st = (sa.cuda_stream, sb.cuda_stream, 0)[j % 3]
+            st = (sa.handle, sb.handle, 0)[j % 3]
             run(st, q0, q0 + 100, gi, gp)
-        torch.cuda.synchronize()
-        assert np.array_equal(gi.cpu().numpy(), ref_i), rep
-        assert np.array_equal(gp.cpu().numpy().view(np.uint32), ref_p.view(np.uint32)), rep
+        synchronize()
+        assert np.array_equal(gi.numpy(), ref_i), rep
+        assert np.array_equal(gp.numpy().view(np.uint32), ref_p.view(np.uint32)), rep
 
 
 def test_update_waits_for_a_queued_plan_run(CEOs):
@@ -92,7 +91,7 @@ def test_update_waits_for_a_queued_plan_run(CEOs):
     search before the update."""
     import ctypes
 
-    import torch
+    from tests._hip import Stream, hip, synchronize
 
     X = _rand(50000, 48, 5)
     Q = _rand(2000, 48, 6)
@@ -106,20 +105,19 @@ def test_update_waits_for_a_queued_plan_run(CEOs):
     plan = s.make_plan(nq, 10, 60, 3000, autotune=False)
     ref_i, ref_p = plan.search(Q, k, sv, c)  # also leaves Q, k, s, c in the plan's buffers
     buf = plan.buffers()
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    st = torch.cuda.Stream()
+    L = hip()
+    st = Stream()
     for rep in range(2):
         # clear the outputs, queue the replay, then update at once (no synchronisation)
         zero = np.zeros((nq, 10), np.int32)
-        assert hip.hipMemcpy(buf["ids"], zero.ctypes.data, zero.nbytes, 1) == 0
-        plan.run(nq, st.cuda_stream)
+        assert L.hipMemcpy(buf["ids"], zero.ctypes.data, zero.nbytes, 1) == 0
+        plan.run(nq, st.handle)
         s.update(_rand(5000, 48, 10 + rep), 5000)
-        torch.cuda.synchronize()
+        synchronize()
         gi = np.zeros((nq, 10), np.int32)
         gp = np.zeros((nq, 10), np.float32)
-        assert hip.hipMemcpy(gi.ctypes.data, buf["ids"], gi.nbytes, 2) == 0
-        assert hip.hipMemcpy(gp.ctypes.data, buf["ips"], gp.nbytes, 2) == 0
+        assert L.hipMemcpy(gi.ctypes.data, buf["ids"], gi.nbytes, 2) == 0
+        assert L.hipMemcpy(gp.ctypes.data, buf["ips"], gp.nbytes, 2) == 0
         assert np.array_equal(gi, ref_i), rep
         assert np.array_equal(gp.view(np.uint32), ref_p.view(np.uint32)), rep
         ref_i, ref_p = plan.search(Q, k, sv, c)  # the updated index, for the next round

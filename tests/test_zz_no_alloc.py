@@ -15,13 +15,15 @@ def _rand(n, d, seed):
     return (rng.standard_normal((n, d)) * rng.lognormal(0, 0.5, (n, 1))).astype(np.float32)
 
 
-def _search(L, idx, fn, dQ, nq, d, k, torch, dev):
-    ids = torch.full((nq, k), -7, dtype=torch.int32, device=dev)
-    ips = torch.full((nq, k), -7.0, dtype=torch.float32, device=dev)
-    rc = getattr(L, fn)(idx.h, dQ.data_ptr(), nq, d, k, ids.data_ptr(), ips.data_ptr(), None)
+def _search(L, idx, fn, dQ, nq, d, k):
+    from tests._hip import DeviceArray, synchronize
+
+    ids = DeviceArray(np.full((nq, k), -7, np.int32))
+    ips = DeviceArray(np.full((nq, k), -7.0, np.float32))
+    rc = getattr(L, fn)(idx.h, dQ.ptr, nq, d, k, ids.ptr, ips.ptr, None)
     assert rc == 0, L.ceos_last_error()
-    torch.cuda.synchronize()
-    return ids.cpu().numpy(), ips.cpu().numpy()
+    synchronize()
+    return ids.numpy(), ips.numpy()
 
 
 @pytest.mark.parametrize("n,d,D,R,m,s,c,k,nq", [
@@ -30,44 +32,40 @@ def _search(L, idx, fn, dQ, nq, d, k, torch, dev):
     (30000, 960, 1024, 8, 500, 80, 5000, 100, 400),    # the largest per-workgroup share
 ])
 def test_noalloc_equals_allocating_search(n, d, D, R, m, s, c, k, nq):
-    import torch
-
     from ceos_amd import capi
+    from tests._hip import DeviceArray
 
-    dev = torch.device("cuda:0")
     L = capi.lib()
     X = _rand(n, d, 1)
     Q = _rand(nq, d, 2)
     idx = capi.DeviceIndex(n, d)
     idx.set_index_param(D, R, m, -1, 42)
-    dX = torch.from_numpy(X).to(dev)
-    dQ = torch.from_numpy(Q).to(dev)
-    idx.build_device(dX.data_ptr(), n)
+    dX = DeviceArray(X)
+    dQ = DeviceArray(Q)
+    idx.build_device(dX.ptr, n)
     idx.set_query_param(s, m, c)
     for nqi in (nq, nq // 3 + 1):  # two calls in a row, the second smaller
         print(f"noalloc case n={n} d={d} nq={nqi}", flush=True)
-        ri, rp = _search(L, idx, "ceos_search_coceos_device", dQ, nqi, d, k, torch, dev)
-        gi, gp = _search(L, idx, "ceos_search_coceos_device_noalloc", dQ, nqi, d, k, torch, dev)
+        ri, rp = _search(L, idx, "ceos_search_coceos_device", dQ, nqi, d, k)
+        gi, gp = _search(L, idx, "ceos_search_coceos_device_noalloc", dQ, nqi, d, k)
         assert np.array_equal(gi, ri)
         assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32))
 
 
 def test_noalloc_rejects_oversized_scratch():
-    import torch
-
     from ceos_amd import capi
+    from tests._hip import DeviceArray
 
-    dev = torch.device("cuda:0")
     L = capi.lib()
     n, d = 5000, 16
     X = _rand(n, d, 3)
     idx = capi.DeviceIndex(n, d)
     idx.set_index_param(1024, 4, 4096, -1, 42)
-    dX = torch.from_numpy(X).to(dev)
-    idx.build_device(dX.data_ptr(), n)
+    dX = DeviceArray(X)
+    idx.build_device(dX.ptr, n)
     idx.set_query_param(4096, 4096, 100)  # 2^24 probed positions: 256 MiB for one workgroup
-    dQ = torch.from_numpy(X[:4]).to(dev)
-    ids = torch.zeros((4, 10), dtype=torch.int32, device=dev)
-    ips = torch.zeros((4, 10), dtype=torch.float32, device=dev)
-    rc = L.ceos_search_coceos_device_noalloc(idx.h, dQ.data_ptr(), 4, d, 10, ids.data_ptr(), ips.data_ptr(), None)
+    dQ = DeviceArray(X[:4])
+    ids = DeviceArray(shape=(4, 10), dtype=np.int32)
+    ips = DeviceArray(shape=(4, 10), dtype=np.float32)
+    rc = L.ceos_search_coceos_device_noalloc(idx.h, dQ.ptr, 4, d, 10, ids.ptr, ips.ptr, None)
     assert rc == 4 and b"static scratch" in L.ceos_last_error()
