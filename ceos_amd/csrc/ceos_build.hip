@@ -245,7 +245,7 @@ hipError_t launch_sample_thresholds(const float* sample, int64_t S, int DR, int 
 // (no atomics), and flushed when the stage may overflow: every lane then takes staged
 // hits and reserves their positions with global atomics issued back to back, so one
 // round trip serves up to kStage hits instead of one divergent round trip per hit.
-constexpr int kStage = 512;  // staged hits per wave
+constexpr int kStage = 256;  // staged hits per wave (5 workgroups of 4 waves fit a CU's LDS)
 
 template <int L>
 __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const float* __restrict__ X, int64_t n,
@@ -263,23 +263,29 @@ __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const fl
     uint32_t* st_list = reinterpret_cast<uint32_t*>(lds_all + kBuildWaves * P::kLdsWords) + wave * 3 * kStage;
     uint32_t* st_id = st_list + kStage;
     uint32_t* st_val = st_id + kStage;
-    const int r = blockIdx.y;
+    // Workgroup blockIdx.x = block * R + repeat: the R workgroups of a point block are
+    // dispatched together (on different XCDs) and read the same rows at about the same
+    // time, so the memory-side cache serves the repeats after the first instead of R
+    // sweeps over the points.
+    const int r = (int)(blockIdx.x % (unsigned)dm.R);
+    const int64_t bx = blockIdx.x / (unsigned)dm.R, gxr = gridDim.x / (unsigned)dm.R;
     const int W = sign_words(L);
     const uint32_t* sg = signs + (size_t)r * 3 * W;
 
+    // an inactive list (or a column past n_proj) gets a NaN threshold: no value passes it
     float th_c[P::OE], th_f[P::OE];
-    uint64_t act_c = 0, act_f = 0;
+    bool any_active = false;
 #pragma unroll
     for (int reg = 0; reg < P::OE; ++reg) {
         const int j = reg * kWave + ln;
-        th_c[reg] = th_f[reg] = 0.0f;
+        th_c[reg] = th_f[reg] = __builtin_nanf("");
         if (j < dm.D) {
             const int c = r * dm.D + j;
-            if (!active || active[2 * c]) { th_c[reg] = thr[2 * c]; act_c |= 1ull << reg; }
-            if (!active || active[2 * c + 1]) { th_f[reg] = thr[2 * c + 1]; act_f |= 1ull << reg; }
+            if (!active || active[2 * c]) { th_c[reg] = thr[2 * c]; any_active = true; }
+            if (!active || active[2 * c + 1]) { th_f[reg] = thr[2 * c + 1]; any_active = true; }
         }
     }
-    if (!__any((act_c | act_f) != 0ull)) return;
+    if (!__any(any_active)) return;
 
     uint32_t staged = 0;  // wave-uniform
     auto flush = [&]() {
@@ -303,7 +309,7 @@ __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const fl
         staged = 0;
     };
 
-    for (int64_t p = (int64_t)blockIdx.x * kBuildWaves + wave; p < n; p += (int64_t)gridDim.x * kBuildWaves) {
+    for (int64_t p = bx * kBuildWaves + wave; p < n; p += gxr * kBuildWaves) {
         float xr[P::XE];
         P::load_x(X + p * dm.d, dm.d, xr, ln);
         float y[P::OE];
@@ -312,8 +318,8 @@ __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const fl
         for (int reg = 0; reg < P::OE; ++reg) {
             const float v = y[reg];
             const uint32_t c = (uint32_t)(r * dm.D + reg * kWave + ln);
-            const bool hc = ((act_c >> reg) & 1ull) && v >= th_c[reg];
-            const bool hf = ((act_f >> reg) & 1ull) && -v >= th_f[reg];
+            const bool hc = v >= th_c[reg];
+            const bool hf = -v >= th_f[reg];
             const uint64_t mc = __ballot(hc), mf = __ballot(hf);
             if (mc | mf) {  // wave-uniform
                 if (staged + (uint32_t)(__popcll(mc) + __popcll(mf)) > (uint32_t)kStage) flush();  // <= 128 fit after
@@ -345,8 +351,8 @@ hipError_t launch_scan_append(const IndexDims& dm, const float* X, int64_t n, co
     CEOS_SWITCH_L(dm.L, {
         const size_t lds = (size_t)kBuildWaves * (Proj<LL>::kLdsWords * sizeof(float) + 3 * kStage * 4);
         allow_lds(k_scan_append<LL>, lds);
-        hipLaunchKernelGGL(k_scan_append<LL>, dim3(gx, dm.R), dim3(kBuildNT), lds, st, dm, X, n, signs, thr,
-                           active, base, cap, count, ent);
+        hipLaunchKernelGGL(k_scan_append<LL>, dim3((unsigned)(gx * dm.R)), dim3(kBuildNT), lds, st, dm, X, n, signs,
+                           thr, active, base, cap, count, ent);
     });
     return hipGetLastError();
 }

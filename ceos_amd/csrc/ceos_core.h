@@ -395,10 +395,10 @@ struct Core {
     }
 
     // Copies n rows (host or device pointer) into a fresh ring starting at slot 0.
+    // A ring of the same capacity is reused (a rebuild of the same shape allocates nothing);
+    // the int8 copies are kept for quantize_rows to overwrite or resize.
     void load_points(const float* src, int64_t n, int64_t capacity, bool src_device, hipStream_t s_) {
-        Xq.release();
-        Xqs.release();
-        X.alloc((size_t)capacity * d);
+        if (X.n != (size_t)capacity * d) X.alloc((size_t)capacity * d);
         cap = capacity; head = 0; base = 0; live = (uint32_t)n;
         HIP_CHECK(hipMemcpyAsync(X.p, src, (size_t)n * d * 4,
                                  src_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s_));
@@ -487,8 +487,8 @@ struct Core {
             if (iter >= 2) throw DevError("bucket construction did not converge");
             if (!hthr.empty()) HIP_CHECK(hipMemcpy(thr.p, hthr.data(), nl * 4, hipMemcpyHostToDevice));
         }
-        bkt.alloc((size_t)nl * m);
-        bsize.alloc(nl);
+        bkt.ensure((size_t)nl * m);
+        bsize.ensure(nl);
         phase_mark(2, s_);
         HIP_CHECK(ceos::launch_list_select(nl, m, dbase.p, dcap.p, dcount.p, ent.p, bkt.p, bsize.p, s_));
         phase_mark(3, s_);

@@ -81,6 +81,21 @@ struct Proj<L, true> {
     // x (d <= F floats) -> registers in the b = 0 layout (zero padded).
     __device__ static void load_x(const float* __restrict__ x, int d, float (&xr)[XE], int ln) {
         const int i0 = ln * E;
+        // uniform fast path: with d a multiple of E and x 16-byte aligned, each lane's E
+        // coordinates are all inside the row or all past it (one branch for the wave)
+        if (E % 4 == 0 && d % E == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+            if (i0 < d) {
+#pragma unroll
+                for (int r = 0; r < E; r += 4) {
+                    float4 v = *reinterpret_cast<const float4*>(x + i0 + r);
+                    xr[r] = v.x; xr[r + 1] = v.y; xr[r + 2] = v.z; xr[r + 3] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < E; ++r) xr[r] = 0.0f;
+            }
+            return;
+        }
         if (i0 + E <= d && ((reinterpret_cast<uintptr_t>(x + i0) & 15) == 0) && (E % 4 == 0)) {
 #pragma unroll
             for (int r = 0; r < E; r += 4) {
@@ -109,12 +124,21 @@ struct Proj<L, true> {
         }
     }
 
+    // pad(elem(ln, r, b)) = pad(elem(ln, 0, b)) + r * step(b): the register index occupies
+    // element bits [b, b + e), which are clear in elem(ln, 0, b), and for b < e the lane
+    // bits sit at and above bit e, so the pad term (i >> e) splits the same way. One base
+    // address per layout and immediate offsets per register.
+    __device__ static __forceinline__ int step(int b) { return (1 << b) + ((1 << b) >> e); }
     __device__ static __forceinline__ void relayout(float (&y)[E], float* lds, int ln, int bf, int bt) {
+        float* wf = lds + pad(elem(ln, 0, bf));
+        const int sf = step(bf);
 #pragma unroll
-        for (int r = 0; r < E; ++r) lds[pad(elem(ln, r, bf))] = y[r];
+        for (int r = 0; r < E; ++r) wf[r * sf] = y[r];
         wave_sync();
+        const float* rt = lds + pad(elem(ln, 0, bt));
+        const int stt = step(bt);
 #pragma unroll
-        for (int r = 0; r < E; ++r) y[r] = lds[pad(elem(ln, r, bt))];
+        for (int r = 0; r < E; ++r) y[r] = rt[r * stt];
         wave_sync();
     }
 
