@@ -303,7 +303,14 @@ def main():
         torch.cuda.synchronize()
         builds.append(time.perf_counter() - t0)
     build_s = builds[-1]
-    log(f"build {build_s * 1e3:.1f} ms ({n / build_s / 1e6:.1f} Mpts/s)")
+    pt = (ctypes.c_double * 5)()  # ceos_phase_times: copy, project, select, total (ms), index GB
+    capi.check(L.ceos_get_build_times(h, ctypes.cast(pt, ctypes.c_void_p)))
+    rs = (ctypes.c_int * 3)()
+    capi.check(L.ceos_get_build_rescans(h, ctypes.byref(rs, 0), ctypes.byref(rs, 4), ctypes.byref(rs, 8)))
+    build_phases = {"copy_ms": round(pt[0], 3), "project_ms": round(pt[1], 3), "select_ms": round(pt[2], 3),
+                    "rescans": {"lists_under": rs[0], "lists_over": rs[1], "passes": rs[2]}}
+    log(f"build {build_s * 1e3:.1f} ms ({n / build_s / 1e6:.1f} Mpts/s): copy {pt[0]:.3f} ms, "
+        f"sample + scan {pt[1]:.3f} ms, list select {pt[2]:.3f} ms; rescans {list(rs)}")
     index.n_probed_vectors = w.s
     index.n_probed_points = w.m
     index.n_cand = w.c
@@ -582,7 +589,7 @@ def main():
     if index.rerank_bounds:
         build_bytes += (5.0 * d + 16) * n
     build = {"mpts_s": round(n / build_s / 1e6, 2), "ms": round(build_s * 1e3, 2),
-             "adds_per_point": adds_pt,
+             "phases": build_phases, "adds_per_point": adds_pt,
              "valu": {"achieved": round(adds_pt * n / build_s / 1e12, 3), "peak": VALU_ADD_PEAK_T,
                       "unit": "T add/s", "frac": round(adds_pt * n / build_s / 1e12 / VALU_ADD_PEAK_T, 4)},
              "hbm": {"achieved": round(build_bytes / build_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

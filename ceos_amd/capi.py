@@ -64,6 +64,9 @@ def lib():
             "ceos_stream_update": (i32, [vp, P, i64, i32, i32]),
             "ceos_stream_get_build_times": (i32, [vp, P]),
             "ceos_get_build_times": (i32, [vp, P]),
+            "ceos_set_build_budget": (i32, [vp, i64]),
+            "ceos_get_build_rescans": (i32, [vp, P, P, P]),
+            "ceos_stream_set_build_budget": (i32, [vp, i64]),
             "ceos_get_search_times": (i32, [vp, P]),
         }
         for name, (res, args) in sig.items():

@@ -151,8 +151,9 @@ hipError_t launch_transpose(const float* in, int64_t rows, int64_t cols, float* 
 // kBuildNT per-thread maxima is at most the rank-th largest value (rank <= kBuildNT);
 // likewise for the negated values with the per-thread minima. A second read of the column
 // collects every value at or above the first bound and every negated value at or above the
-// second (a few hundred each) in LDS, and the rank-th largest of each collection is exact.
-// A side whose collection exceeds kThrCollect falls back to a radix select over the column.
+// second (a few hundred each) in LDS, and the rank-th largest of each collection is exact
+// (a collection of exactly rank values: its smallest). A side whose collection exceeds
+// kThrCollect falls back to a radix select over the column.
 constexpr int kThrCollect = 2048;
 
 __global__ __launch_bounds__(kBuildNT) void k_sample_thresholds(const float* __restrict__ sample, int64_t S,
@@ -161,6 +162,7 @@ __global__ __launch_bounds__(kBuildNT) void k_sample_thresholds(const float* __r
     __shared__ float ext[2][kBuildNT];       // per-thread maxima, negated minima
     __shared__ float coll[2][kThrCollect];   // collected values, negated values
     __shared__ uint32_t counter[2];
+    __shared__ uint32_t kmin[2];
     __shared__ float bound[2];
     const int col = blockIdx.x;
     const float* v = sample + (int64_t)col * S;
@@ -173,6 +175,7 @@ __global__ __launch_bounds__(kBuildNT) void k_sample_thresholds(const float* __r
     ext[0][threadIdx.x] = mx;
     ext[1][threadIdx.x] = nmn;
     if (threadIdx.x == 0) counter[0] = counter[1] = 0;
+    if (threadIdx.x == 0) kmin[0] = kmin[1] = 0xffffffffu;
     __syncthreads();
     for (int side = 0; side < 2; ++side) {
         uint64_t prefix;
@@ -201,7 +204,14 @@ __global__ __launch_bounds__(kBuildNT) void k_sample_thresholds(const float* __r
         const bool neg = side == 1;
         uint64_t prefix;
         int shift;
-        if (cnt <= kThrCollect) {
+        if (cnt == rank) {  // every collected value is in the top rank (cnt >= rank always)
+            uint32_t mn = 0xffffffffu;
+            for (int i = threadIdx.x; i < cnt; i += kBuildNT) mn = min(mn, fkey(c[i]));
+            atomicMin(&kmin[side], mn);
+            __syncthreads();
+            prefix = (uint64_t)kmin[side] << 32;
+            shift = 32;
+        } else if (cnt <= kThrCollect) {
             radix_select<kBuildNT>(cnt, rank, [&](int i) -> uint64_t { return (uint64_t)fkey(c[i]) << 32; }, sh,
                                    prefix, shift, 32);
         } else {  // rare: too many values at the bound
@@ -248,13 +258,14 @@ hipError_t launch_sample_thresholds(const float* sample, int64_t S, int DR, int 
 constexpr int kStage = 256;  // staged hits per wave (5 workgroups of 4 waves fit a CU's LDS)
 
 template <int L>
-__global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const float* __restrict__ X, int64_t n,
+__global__ __launch_bounds__(kBuildNT) __attribute__((amdgpu_waves_per_eu(L <= 10 ? 5 : 1))) void k_scan_append(IndexDims dm, const float* __restrict__ X, int64_t n,
                                                           const uint32_t* __restrict__ signs,
                                                           const float* __restrict__ thr,
                                                           const uint8_t* __restrict__ active,
                                                           const int64_t* __restrict__ base,
                                                           const int32_t* __restrict__ cap,
-                                                          int32_t* __restrict__ count, Entry* __restrict__ ent) {
+                                                          int32_t* __restrict__ count, Entry* __restrict__ ent,
+                                                          float* __restrict__ copy_to) {
     using P = Proj<L>;
     extern __shared__ float lds_all[];
     const int wave = threadIdx.x / kWave, ln = threadIdx.x % kWave;
@@ -271,6 +282,7 @@ __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const fl
     const int64_t bx = blockIdx.x / (unsigned)dm.R, gxr = gridDim.x / (unsigned)dm.R;
     const int W = sign_words(L);
     const uint32_t* sg = signs + (size_t)r * 3 * W;
+    const typename P::Signs sw = P::load_signs(sg, ln);
 
     // an inactive list (or a column past n_proj) gets a NaN threshold: no value passes it
     float th_c[P::OE], th_f[P::OE];
@@ -285,7 +297,7 @@ __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const fl
             if (!active || active[2 * c + 1]) { th_f[reg] = thr[2 * c + 1]; any_active = true; }
         }
     }
-    if (!__any(any_active)) return;
+    if (!__any(any_active) && !copy_to) return;
 
     uint32_t staged = 0;  // wave-uniform
     auto flush = [&]() {
@@ -309,11 +321,18 @@ __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const fl
         staged = 0;
     };
 
-    for (int64_t p = bx * kBuildWaves + wave; p < n; p += gxr * kBuildWaves) {
+    // pr = p mod R, kept up to date without 64-bit divisions
+    const int64_t pstep = gxr * kBuildWaves;
+    const int rstep = (int)(pstep % dm.R);
+    int pr = (int)((bx * kBuildWaves + wave) % dm.R);
+    for (int64_t p = bx * kBuildWaves + wave; p < n; p += pstep) {
         float xr[P::XE];
         P::load_x(X + p * dm.d, dm.d, xr, ln);
+        if (copy_to && pr == r) P::store_x(copy_to + p * dm.d, dm.d, xr, ln);  // wave-uniform
+        pr += rstep;
+        if (pr >= dm.R) pr -= dm.R;
         float y[P::OE];
-        P::run(xr, sg, lds, ln, y);
+        P::run(xr, sw, lds, ln, y);
 #pragma unroll
         for (int reg = 0; reg < P::OE; ++reg) {
             const float v = y[reg];
@@ -345,14 +364,14 @@ __global__ __launch_bounds__(kBuildNT) void k_scan_append(IndexDims dm, const fl
 
 hipError_t launch_scan_append(const IndexDims& dm, const float* X, int64_t n, const uint32_t* signs,
                               const float* thr, const uint8_t* active, const int64_t* base,
-                              const int32_t* cap, int32_t* count, Entry* ent, hipStream_t st) {
+                              const int32_t* cap, int32_t* count, Entry* ent, hipStream_t st, float* copy_to) {
     if (n <= 0) return hipSuccess;
     int gx = (int)std::min<int64_t>((n + kBuildWaves - 1) / kBuildWaves, std::max(1, 2048 / dm.R) * 4);
     CEOS_SWITCH_L(dm.L, {
         const size_t lds = (size_t)kBuildWaves * (Proj<LL>::kLdsWords * sizeof(float) + 3 * kStage * 4);
         allow_lds(k_scan_append<LL>, lds);
         hipLaunchKernelGGL(k_scan_append<LL>, dim3((unsigned)(gx * dm.R)), dim3(kBuildNT), lds, st, dm, X, n, signs,
-                           thr, active, base, cap, count, ent);
+                           thr, active, base, cap, count, ent, copy_to);
     });
     return hipGetLastError();
 }

@@ -99,6 +99,32 @@ struct DBuf {
     }
 };
 
+// Pinned host buffer (device copies into it run without a staging round trip).
+template <typename T>
+struct HBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    HBuf() = default;
+    HBuf(const HBuf&) = delete;
+    HBuf& operator=(const HBuf&) = delete;
+    ~HBuf() { release(); }
+    void ensure(size_t count) {
+        if (count <= n) return;
+        release();
+        if (hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            throw MemError("pinned host allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
 // src/CEOs.h:76-79
 inline int fht_dim(int n_proj, int d) {
     if (n_proj < d) return 1 << (int)std::ceil(std::log2((double)d));
@@ -227,6 +253,13 @@ struct Core {
     DBuf<float> bw_rows, bw_sample, bw_thr;
     DBuf<int64_t> bw_base;
     DBuf<int32_t> bw_cap, bw_count;
+    HBuf<int32_t> bw_hcount;
+    DBuf<uint8_t> bw_active;
+    // list rescans of a build (rare): entries per rescan range (at least n)
+    int64_t rescan_budget_entries = int64_t(1) << 27;
+    struct RescanStats {
+        int under = 0, over = 0, ranges = 0;
+    } rescan;
     DBuf<Entry> bw_ent;
     int64_t est_n = 0;
 
@@ -321,6 +354,12 @@ struct Core {
         for (auto& e : kev)
             for (hipEvent_t x : e) (void)hipEventDestroy(x);
         if (st_) (void)hipStreamDestroy(st_);
+        if (aux_) {
+            (void)hipStreamSynchronize(aux_);
+            (void)hipStreamDestroy(aux_);
+        }
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        if (join_ev) (void)hipEventDestroy(join_ev);
     }
     static constexpr size_t kMaxTimedLaunches = 4096;
     // Attaches the next event triple to a search launch when timing is on.
@@ -340,6 +379,27 @@ struct Core {
     hipStream_t st() {
         if (!st_) HIP_CHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         return st_;
+    }
+    // A second stream for build work that can overlap the bucket scan (default priority).
+    hipStream_t aux_ = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool aux_pending = false;
+    hipStream_t aux() {
+        if (!aux_) HIP_CHECK(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+        if (!fork_ev) HIP_CHECK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+        if (!join_ev) HIP_CHECK(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+        return aux_;
+    }
+    // Makes s_ wait for the work put on the second stream (no host wait).
+    void join_aux(hipStream_t s_) {
+        if (!aux_pending) return;
+        HIP_CHECK(hipStreamWaitEvent(s_, join_ev, 0));
+        aux_pending = false;
+    }
+    // Host wait for the second stream (error paths: its work may read caller memory).
+    void drain_aux() {
+        if (aux_) (void)hipStreamSynchronize(aux_);
+        aux_pending = false;
     }
 
     int DR() const { return D * R; }
@@ -369,7 +429,7 @@ struct Core {
         Xq.release(); Xqs.release();
         scr.release(); estP.release();
         bw_rows.release(); bw_sample.release(); bw_thr.release(); bw_base.release(); bw_cap.release();
-        bw_count.release(); bw_ent.release();
+        bw_count.release(); bw_ent.release(); bw_hcount.release(); bw_active.release();
         bits.clear();
         cap = head = 0; base = live = 0; est_n = 0;
         built = false;
@@ -404,6 +464,12 @@ struct Core {
                                  src_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s_));
     }
 
+    // A ring for n rows that the first bucket scan fills (build_buckets with src).
+    void reserve_points(int64_t n, int64_t capacity) {
+        if (X.n != (size_t)capacity * d) X.alloc((size_t)capacity * d);
+        cap = capacity; head = 0; base = 0; live = (uint32_t)n;
+    }
+
     // Centering (src/streamCEOs.cpp:22-34): the column mean of rows 0..n-1 of X (ring slots
     // 0..n-1) is computed on the device, subtracted from those rows, kept on the device
     // for later inserts and copied to `center` on the host (index files, get_center).
@@ -419,16 +485,24 @@ struct Core {
     }
 
     // coCEOs buckets over rows 0..n-1 of X (ring slots 0..n-1): see ceos_build.hip.
-    void build_buckets(int64_t n, hipStream_t s_) {
+    // src (optional, a device copy of those rows, 16-byte aligned): the scans read it
+    // instead, and the first one also stores the rows into X (after reserve_points), so
+    // the points are read from HBM once for both.
+    void build_buckets(int64_t n, hipStream_t s_, const float* src = nullptr) {
         const int DR_ = DR();
         const int nl = 2 * DR_;
         const IndexDims dm = dims();
         if (m > n) throw ArgError("top_m must not exceed the number of points");
+        const float* rows = src ? src : X.p;
+        float* copy_to = src ? X.p : nullptr;
 
-        // sample size: about 32 n/m points (at least 32768); the threshold is the sample
-        // value of rank 2.5 m S/n + 8 (about 90), so about 2.75 m points pass it and a list
-        // under m or over its 4m + 256 cap (both checked; rescanned) is a > 5 sigma event
-        int64_t S_target = std::max<int64_t>(32768, (32 * n + m - 1) / m);
+        // sample size: about 16 n/m points (at least 32768), so a list's top m hold about
+        // lambda = m S/n >= 16 sample points. The threshold is the sample value of rank
+        // k = 2.5 lambda + 8 (about 48); the points passing it number about n/S * Gamma(k).
+        // A list under m (P(Poisson(lambda) >= k), about 1e-10 at lambda = 16) or over its
+        // space n/S (k + 8 sqrt(k) + 16) + 256 (an 8-sigma Gamma tail) is checked for and
+        // rescanned.
+        int64_t S_target = std::max<int64_t>(32768, (16 * n + m - 1) / m);
         S_target = std::min<int64_t>(S_target, std::min<int64_t>((int64_t(2) << 30) / (4LL * DR_), 65535LL * 64));
         S_target = std::max<int64_t>(S_target, 1);
         const int64_t stride = std::max<int64_t>(1, n / S_target);
@@ -442,57 +516,96 @@ struct Core {
             // project the sample row-major (coalesced), then transpose to one column per list
             bw_rows.ensure((size_t)DR_ * S);
             bw_sample.ensure((size_t)DR_ * S);
-            HIP_CHECK(ceos::launch_project_rows(dm, X.p, 0, stride, S, signs.p, bw_rows.p, DR_, false, s_));
+            HIP_CHECK(ceos::launch_project_rows(dm, rows, 0, stride, S, signs.p, bw_rows.p, DR_, false, s_));
             HIP_CHECK(ceos::launch_transpose(bw_rows.p, S, DR_, bw_sample.p, s_));
             HIP_CHECK(ceos::launch_sample_thresholds(bw_sample.p, S, DR_, rank, thr.p, s_));
         }
+        // One scan appends every point that passes its list's threshold; the lists are then
+        // selected without waiting for the counts (the common case ends here, with one
+        // synchronisation). A list left under m entries (threshold too high for it) is
+        // rescanned with no threshold and capacity n; one over its capacity, with capacity
+        // = its count. Rescans cover contiguous ranges of lists whose capacities fit the
+        // entry budget, the lists outside a range inactive, so the memory stays bounded
+        // however many lists the strided sample misjudged.
+        const int32_t cap0 = (int32_t)std::min<int64_t>(
+            n, (int64_t)std::ceil((double)n / (double)S * (rank + 8.0 * std::sqrt((double)rank) + 16.0)) + 256);
         std::vector<int64_t> hbase(nl);
-        std::vector<int32_t> hcap(nl, (int32_t)std::min<int64_t>(n, 4LL * m + 256));
-        std::vector<int32_t> hcount(nl);
-        std::vector<float> hthr;
+        std::vector<int32_t> hcap(nl, cap0);
         DBuf<int64_t>& dbase = bw_base;
         DBuf<int32_t>& dcap = bw_cap;
         DBuf<int32_t>& dcount = bw_count;
         DBuf<Entry>& ent = bw_ent;
+        HBuf<int32_t>& hcount = bw_hcount;
         dbase.ensure(nl);
         dcap.ensure(nl);
         dcount.ensure(nl);
-        for (int iter = 0;; ++iter) {
-            int64_t total = 0;
-            for (int l = 0; l < nl; ++l) { hbase[l] = total; total += hcap[l]; }
-            ent.ensure((size_t)total);
-            HIP_CHECK(hipMemcpyAsync(dbase.p, hbase.data(), nl * 8, hipMemcpyHostToDevice, s_));
-            HIP_CHECK(hipMemcpyAsync(dcap.p, hcap.data(), nl * 4, hipMemcpyHostToDevice, s_));
-            HIP_CHECK(ceos::launch_fill_i32(dcount.p, 0, nl, s_));
-            HIP_CHECK(ceos::launch_scan_append(dm, X.p, n, signs.p, thr.p, nullptr, dbase.p, dcap.p, dcount.p,
-                                               ent.p, s_));
-            HIP_CHECK(hipMemcpyAsync(hcount.data(), dcount.p, nl * 4, hipMemcpyDeviceToHost, s_));
-            HIP_CHECK(hipStreamSynchronize(s_));
-            bool redo = false;
-            for (int l = 0; l < nl; ++l) {
-                if (hcount[l] < m) {
-                    if (hthr.empty()) {
-                        hthr.resize(nl);
-                        HIP_CHECK(hipMemcpy(hthr.data(), thr.p, nl * 4, hipMemcpyDeviceToHost));
-                    }
-                    hthr[l] = -INFINITY;  // accept every point for this list
-                    hcap[l] = (int32_t)n;
-                    redo = true;
-                } else if (hcount[l] > hcap[l]) {
-                    hcap[l] = hcount[l];
-                    redo = true;
-                }
-            }
-            if (!redo) break;
-            if (iter >= 2) throw DevError("bucket construction did not converge");
-            if (!hthr.empty()) HIP_CHECK(hipMemcpy(thr.p, hthr.data(), nl * 4, hipMemcpyHostToDevice));
-        }
+        hcount.ensure(nl);
         bkt.ensure((size_t)nl * m);
         bsize.ensure(nl);
-        phase_mark(2, s_);
-        HIP_CHECK(ceos::launch_list_select(nl, m, dbase.p, dcap.p, dcount.p, ent.p, bkt.p, bsize.p, s_));
-        phase_mark(3, s_);
-        HIP_CHECK(hipStreamSynchronize(s_));
+        auto scan_select = [&](int lo, int hi, const uint8_t* active) {
+            int64_t total = 0;
+            for (int l = lo; l < hi; ++l) { hbase[l] = total; total += hcap[l]; }
+            ent.ensure((size_t)total);
+            HIP_CHECK(hipMemcpyAsync(dbase.p + lo, hbase.data() + lo, (size_t)(hi - lo) * 8, hipMemcpyHostToDevice, s_));
+            HIP_CHECK(hipMemcpyAsync(dcap.p + lo, hcap.data() + lo, (size_t)(hi - lo) * 4, hipMemcpyHostToDevice, s_));
+            HIP_CHECK(ceos::launch_fill_i32(dcount.p, 0, nl, s_));
+            HIP_CHECK(ceos::launch_scan_append(dm, rows, n, signs.p, thr.p, active, dbase.p, dcap.p, dcount.p,
+                                               ent.p, s_, copy_to));
+            copy_to = nullptr;  // rescans leave X as it is
+            HIP_CHECK(hipMemcpyAsync(hcount.p + lo, dcount.p + lo, (size_t)(hi - lo) * 4, hipMemcpyDeviceToHost, s_));
+            phase_mark(2, s_);
+            HIP_CHECK(ceos::launch_list_select(hi - lo, m, dbase.p + lo, dcap.p + lo, dcount.p + lo, ent.p,
+                                               bkt.p + (size_t)lo * m, bsize.p + lo, s_));
+            phase_mark(3, s_);
+            HIP_CHECK(hipStreamSynchronize(s_));
+        };
+        scan_select(0, nl, nullptr);
+
+        std::vector<uint8_t> bad(nl, 0);
+        std::vector<float> hthr;
+        bool any_bad = false;
+        rescan = RescanStats{};
+        for (int l = 0; l < nl; ++l) {
+            if (hcount.p[l] < m) {
+                ++rescan.under;
+                if (hthr.empty()) {
+                    hthr.resize(nl);
+                    HIP_CHECK(hipMemcpy(hthr.data(), thr.p, nl * 4, hipMemcpyDeviceToHost));
+                }
+                hthr[l] = -INFINITY;  // accept every point for this list
+                hcap[l] = (int32_t)n;
+                bad[l] = any_bad = 1;
+            } else if (hcount.p[l] > hcap[l]) {
+                ++rescan.over;
+                hcap[l] = hcount.p[l];  // the same threshold admits exactly this many again
+                bad[l] = any_bad = 1;
+            }
+        }
+        if (!any_bad) return;
+        if (!hthr.empty()) HIP_CHECK(hipMemcpy(thr.p, hthr.data(), nl * 4, hipMemcpyHostToDevice));
+        const int64_t budget = std::max<int64_t>(rescan_budget_entries, n);
+        std::vector<uint8_t> hact(nl);
+        bw_active.ensure(nl);
+        for (int lo = 0; lo < nl;) {
+            if (!bad[lo]) { ++lo; continue; }
+            // [lo, hi): from a misjudged list, as far as the budget allows, ending on one
+            int hi = lo + 1, end = lo + 1;
+            int64_t total = hcap[lo];
+            while (hi < nl && total + hcap[hi] <= budget) {
+                total += hcap[hi];
+                ++hi;
+                if (bad[hi - 1]) end = hi;
+            }
+            hi = end;
+            std::fill(hact.begin(), hact.end(), 0);
+            std::fill(hact.begin() + lo, hact.begin() + hi, 1);
+            HIP_CHECK(hipMemcpyAsync(bw_active.p, hact.data(), nl, hipMemcpyHostToDevice, s_));
+            scan_select(lo, hi, bw_active.p);
+            ++rescan.ranges;
+            for (int l = lo; l < hi; ++l)
+                if (hcount.p[l] < m || hcount.p[l] > hcap[l]) throw DevError("bucket construction did not converge");
+            lo = hi;
+        }
     }
     // Phase record of a build that marked 0 / 1 around its copy and ran build_buckets.
     void finish_build_times(std::chrono::steady_clock::time_point t0) {
@@ -574,6 +687,26 @@ struct Core {
             count = total;
         }
         HIP_CHECK(ceos::launch_quantize_rows(X.p, slot0, count, cap, d, Xq.p, Xqs.p, s_));
+    }
+    // The rerank bounds of rows 0..n-1 made from src (a device copy of those rows, read
+    // only) on the second stream, after the work queued on s_ so far: the memory-bound
+    // pass overlaps the compute-bound bucket scan. join_aux(s_) orders later work after it.
+    void quantize_rows_side(const float* src, int64_t n, hipStream_t s_) {
+        if (!rerank_bounds || d % 4 != 0 || cap <= 0) {
+            Xq.release();
+            Xqs.release();
+            return;
+        }
+        if (Xq.n != (size_t)cap * d || Xqs.n != (size_t)cap) {
+            Xq.alloc((size_t)cap * d);
+            Xqs.alloc((size_t)cap);
+        }
+        hipStream_t a = aux();
+        HIP_CHECK(hipEventRecord(fork_ev, s_));
+        HIP_CHECK(hipStreamWaitEvent(a, fork_ev, 0));
+        HIP_CHECK(ceos::launch_quantize_rows(src, 0, n, n, d, Xq.p, Xqs.p, a));
+        HIP_CHECK(hipEventRecord(join_ev, a));
+        aux_pending = true;
     }
     void set_rerank_bounds(bool on) {
         wait_reads();

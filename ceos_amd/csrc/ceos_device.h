@@ -108,6 +108,26 @@ struct Proj<L, true> {
         }
     }
 
+    // registers in the load_x layout -> x (the first d elements), non-temporal: a copy
+    // that is not read back soon should not evict rows other waves are about to read
+    __device__ static void store_x(float* __restrict__ x, int d, const float (&xr)[XE], int ln) {
+        const int i0 = ln * E;
+        if (E % 4 == 0 && d % E == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+            if (i0 < d) {
+#pragma unroll
+                for (int r = 0; r < E; r += 4) {
+                    typedef float f4 __attribute__((ext_vector_type(4)));
+                    const f4 v = {xr[r], xr[r + 1], xr[r + 2], xr[r + 3]};
+                    __builtin_nontemporal_store(v, reinterpret_cast<f4*>(x + i0 + r));
+                }
+            }
+            return;
+        }
+#pragma unroll
+        for (int r = 0; r < E; ++r)
+            if (i0 + r < d) __builtin_nontemporal_store(xr[r], x + i0 + r);
+    }
+
     template <int b, int t0, int t1>
     __device__ static __forceinline__ void stages(float (&y)[E]) {
 #pragma unroll
@@ -202,20 +222,37 @@ struct Proj<L, true> {
         }
     }
 
-    // sg: 3 rounds x F/32 words for this repeat. Output y[r] = coordinate r*64 + ln.
-    __device__ static void run(const float (&xr)[XE], const uint32_t* __restrict__ sg, float* lds, int ln,
-                               float (&y)[OE]) {
+    // The lane's sign words of the 3 rounds (loop-invariant for a repeat: load once).
+    struct Signs {
+        uint32_t w[3][2];
+    };
+    __device__ static __forceinline__ Signs load_signs(const uint32_t* __restrict__ sg, int ln) {
+        Signs s;
+#pragma unroll
+        for (int rd = 0; rd < 3; ++rd) sign_bits(sg + rd * (F / 32), ln, s.w[rd]);
+        return s;
+    }
+
+    // Output y[r] = coordinate r*64 + ln.
+    __device__ static void run(const float (&xr)[XE], const Signs& sw, float* lds, int ln, float (&y)[OE]) {
 #pragma unroll
         for (int r = 0; r < E; ++r) y[r] = xr[r];
 #pragma unroll 1
         for (int rd = 0; rd < 3; ++rd) {
             if (rd > 0) relayout(y, lds, ln, L - e, 0);
             uint32_t w[2];
-            sign_bits(sg + rd * (F / 32), ln, w);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) w[k] = rd == 0 ? sw.w[0][k] : rd == 1 ? sw.w[1][k] : sw.w[2][k];
 #pragma unroll
             for (int r = 0; r < E; ++r) y[r] = flip_if_zero_bit(y[r], w[r >> 5], r & 31);
             Phase<0, 0>::run(y, lds, ln);
         }
+    }
+
+    // sg: 3 rounds x F/32 words for this repeat.
+    __device__ static void run(const float (&xr)[XE], const uint32_t* __restrict__ sg, float* lds, int ln,
+                               float (&y)[OE]) {
+        run(xr, load_signs(sg, ln), lds, ln, y);
     }
 };
 
@@ -232,6 +269,23 @@ struct Proj<L, false> {
             const int i = r * 64 + ln;
             xr[r] = (i < d) ? x[i] : 0.0f;
         }
+    }
+
+    __device__ static void store_x(float* __restrict__ x, int d, const float (&xr)[XE], int ln) {
+#pragma unroll
+        for (int r = 0; r < XE; ++r) {
+            const int i = r * 64 + ln;
+            if (i < d) __builtin_nontemporal_store(xr[r], x + i);
+        }
+    }
+
+    // (the LDS path reads its sign words per round)
+    struct Signs {
+        const uint32_t* sg;
+    };
+    __device__ static __forceinline__ Signs load_signs(const uint32_t* __restrict__ sg, int) { return Signs{sg}; }
+    __device__ static void run(const float (&xr)[XE], const Signs& sw, float* lds, int ln, float (&y)[OE]) {
+        run(xr, sw.sg, lds, ln, y);
     }
 
     __device__ static void run(const float (&xr)[XE], const uint32_t* __restrict__ sg, float* lds, int ln,

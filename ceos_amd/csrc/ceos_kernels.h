@@ -57,9 +57,12 @@ hipError_t launch_sample_thresholds(const float* sample, int64_t S, int DR, int 
 // Streams all n points, projects them and appends (id, value) to list L when
 // value >= thr[L] (close) / -value >= thr[L] (far). Lists with active[L] == 0 are
 // skipped (active may be null). Writes only while pos < cap[L]; count[L] counts all.
+// copy_to (optional): rows 0..n-1 of X are also copied there (the workgroups of repeat r
+// store the rows p with p mod R = r).
 hipError_t launch_scan_append(const IndexDims& dm, const float* X, int64_t n, const uint32_t* signs,
                               const float* thr, const uint8_t* active, const int64_t* base,
-                              const int32_t* cap, int32_t* count, Entry* ent, hipStream_t st);
+                              const int32_t* cap, int32_t* count, Entry* ent, hipStream_t st,
+                              float* copy_to = nullptr);
 
 // Per list: top-m of ent[base[L] .. base[L]+min(count[L],cap[L])) by (value desc, id asc),
 // written sorted into bkt[L*m ...]; size[L] = min(m, available).

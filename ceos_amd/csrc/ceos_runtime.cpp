@@ -94,12 +94,30 @@ static void build_coceos_impl(ceos_index* h, const float* X, int64_t n, int d, b
     c.dcenter.release();
     c.center.clear();
     c.phase_mark(0, st);
-    c.load_points(X, n, n, dev, st);
+    // a device source without centering: the bucket scan reads it and stores the rows,
+    // one pass over the points instead of a copy and then the scan
+    const bool direct = dev && !center && (reinterpret_cast<uintptr_t>(X) & 15) == 0;
+    if (direct)
+        c.reserve_points(n, n);
+    else
+        c.load_points(X, n, n, dev, st);
     if (center) c.center_rows(n, st);
     c.phase_mark(1, st);
     c.make_signs();
-    c.build_buckets(n, st);
-    c.quantize_rows(0, n, n, st);
+    if (direct) {
+        // the rerank bounds from the caller's rows on the second stream, beside the scan
+        c.quantize_rows_side(X, n, st);
+        try {
+            c.build_buckets(n, st, X);
+        } catch (...) {
+            c.drain_aux();
+            throw;
+        }
+        c.join_aux(st);
+    } else {
+        c.build_buckets(n, st);
+        c.quantize_rows(0, n, n, st);
+    }
     HIP_CHECK(hipStreamSynchronize(st));
     c.finish_build_times(t0);
     c.built = true;
@@ -371,6 +389,34 @@ int ceos_set_search_config(ceos_index* h, int wg_per_cu, int table_log2) {
         std::lock_guard<std::mutex> g(h->core.mu);
         h->core.wg_per_cu = wg_per_cu;
         h->core.table_log2 = table_log2;
+    });
+}
+
+int ceos_set_build_budget(ceos_index* h, int64_t entries) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        if (entries <= 0) throw ArgError("the build budget must be positive");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.rescan_budget_entries = entries;
+    });
+}
+
+int ceos_get_build_rescans(const ceos_index* h, int* under, int* over, int* passes) {
+    return guarded([&] {
+        if (!h || !under || !over || !passes) throw ArgError("null argument");
+        const Core& c = h->core;
+        *under = c.rescan.under;
+        *over = c.rescan.over;
+        *passes = c.rescan.ranges;
+    });
+}
+
+int ceos_stream_set_build_budget(ceos_stream* h, int64_t entries) {
+    return guarded([&] {
+        if (!h) throw ArgError("null handle");
+        if (entries <= 0) throw ArgError("the build budget must be positive");
+        std::lock_guard<std::mutex> g(h->core.mu);
+        h->core.rescan_budget_entries = entries;
     });
 }
 

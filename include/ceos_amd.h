@@ -146,6 +146,18 @@ int ceos_project(ceos_index* h, const float* rows, int64_t n, int d, float* out)
 int ceos_set_search_config(ceos_index* h, int wg_per_cu, int table_log2);
 int ceos_get_search_config(const ceos_index* h, int* wg_per_cu, int* table_log2, int* lds_bytes);
 
+/* Memory bound of a build's list rescans (no reference counterpart). A build selects
+ * each list's top_m from the points passing a threshold taken from a strided sample; a
+ * list the sample misjudged (fewer than top_m passing, or more than the space set aside)
+ * is scanned again. Rescans run over contiguous ranges of lists holding at most
+ * `entries` candidate entries (8 bytes each; at least n_points is used), so a sample that
+ * misjudges many lists costs more passes, not more memory. Default 2^27 (1 GiB). */
+int ceos_set_build_budget(ceos_index* h, int64_t entries);
+int ceos_stream_set_build_budget(ceos_stream* h, int64_t entries);
+/* The last build's rescans: lists left under top_m, lists over their space, and the
+ * rescan passes run (all 0 in the common case). */
+int ceos_get_build_rescans(const ceos_index* h, int* under, int* over, int* passes);
+
 /* Rerank bounds (no reference counterpart; on by default). While on, a coCEOs or
  * streamCEOs index keeps an int8 copy of every row (n * d bytes, a quarter of the FP32
  * rows) with a per-row scale and error bound, built with the index and kept in step by

@@ -121,7 +121,7 @@ def test_ceos_est_small(CEOs, oracle):
 
 def test_buckets_sampled_thresholds(CEOs, oracle):
     """150,000 points: the build samples every 4th point for its per-list thresholds
-    (rank about 90), so the exact per-list selection runs on the candidates that pass
+    (rank about 83), so the exact per-list selection runs on the candidates that pass
     them. Buckets must equal the oracle's, which scans every point."""
     X = _rand(150_000, 24, 11)
     g, o = _build_pair(CEOs, oracle, X, 64, 2, 120)
@@ -137,6 +137,89 @@ def test_buckets_sampled_thresholds_duplicates(CEOs, oracle):
     X[::3] = X[5]
     g, o = _build_pair(CEOs, oracle, X, 32, 2, 100)
     _compare_buckets(g, o)
+
+
+@pytest.mark.parametrize("scale", [100.0, 0.01])
+def test_buckets_misjudging_sample(CEOs, oracle, scale):
+    """The build's strided sample (every `stride`-th point, stride = n // max(32768,
+    16 n / top_m), 4 here) is made unrepresentative: the sampled rows are scaled up (every
+    list's threshold is too high and under top_m points pass it) or down (nearly every
+    point passes: every list overflows its space). Every list is then rescanned; with a
+    budget of 4 n entries the rescans run over many small ranges of lists. Buckets must
+    still equal the oracle's."""
+    from ceos_amd import capi
+
+    n, d, D, R, m = 150_000, 24, 64, 2, 120
+    stride = n // max(32768, -(-16 * n // m))
+    assert stride == 4
+    X = _rand(n, d, 14)
+    X[::stride] *= np.float32(scale)
+    g = CEOs.CEOs(n, d)
+    g.setIndexParam(D, R, m, -1, 42)
+    capi.check(capi.lib().ceos_set_build_budget(g._handle, 4 * n))
+    g.build_coCEOs_Est(X)
+    o = oracle.OracleCEOs(n, d)
+    o.setIndexParam(D, R, m, 8, 42)
+    o.build_coCEOs_Est(X)
+    _compare_buckets(g, o)
+
+
+@pytest.mark.parametrize("misaligned", [False, True])
+def test_device_source_build(CEOs, oracle, misaligned):
+    """A build from device memory: from a 16-byte aligned source the bucket scan reads
+    the caller's rows and stores the index's copy on the way; a misaligned source is
+    copied first. Buckets must equal the oracle's, and searches (exact rerank, and with
+    the int8 bound pass) must give the same bits as an index built from host memory."""
+    from ceos_amd import capi
+    from tests._hip import DeviceArray
+
+    n, d, D, R, m = 50_000, 96, 128, 3, 200
+    X = _rand(n, d, 15)
+    Q = _rand(100, d, 16, sigma=0.0)
+    flat = np.concatenate([np.zeros(1, np.float32), X.ravel()]) if misaligned else X.ravel()
+    dX = DeviceArray(flat)
+    src = dX.ptr + (4 if misaligned else 0)
+    g = CEOs.CEOs(n, d)
+    g.setIndexParam(D, R, m, -1, 42)
+    capi.check(capi.lib().ceos_build_coceos_device(g._handle, src, n, d, None))
+    o = oracle.OracleCEOs(n, d)
+    o.setIndexParam(D, R, m, 8, 42)
+    o.build_coCEOs_Est(X)
+    _compare_buckets(g, o)
+    h = CEOs.CEOs(n, d)
+    h.setIndexParam(D, R, m, -1, 42)
+    h.build_coCEOs_Est(X)
+    for bounds in (False, True):
+        for idx in (g, h):
+            idx.rerank_bounds = bounds
+            idx.n_probed_vectors = 20
+            idx.n_cand = 2000
+        gi, gp = g.search_coCEOs_Est(Q, 10)
+        hi, hp = h.search_coCEOs_Est(Q, 10)
+        assert np.array_equal(gi, hi)
+        assert np.array_equal(_bits(gp), _bits(hp))
+
+
+def test_build_sampled_thresholds_no_rescans(CEOs):
+    """600,000 points, D = 1024, R = 4, top_m = 300: the threshold sample has 33,334
+    points (130 per thread of the per-column kernel) and rank 49, so in about 1% of the
+    8,192 lists the 49 largest sample values fall to 49 different threads and the
+    collected tail holds exactly rank values. Every threshold must still be the sample's
+    rank-th value: no list may end under top_m or over its space (below 1e-9 per list by
+    the sample's statistics), which a wrong threshold shows as rescans."""
+    from ceos_amd import capi
+
+    n, d = 600_000, 32
+    X = _rand(n, d, 17)
+    g = CEOs.CEOs(n, d)
+    g.setIndexParam(1024, 4, 300, -1, 42)
+    g.build_coCEOs_Est(X)
+    import ctypes
+
+    r = (ctypes.c_int * 3)()
+    capi.check(capi.lib().ceos_get_build_rescans(g._handle, ctypes.byref(r, 0), ctypes.byref(r, 4),
+                                                 ctypes.byref(r, 8)))
+    assert list(r) == [0, 0, 0]
 
 
 def test_buckets_large_top_m_exact_sample(CEOs, oracle):
