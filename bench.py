@@ -620,7 +620,9 @@ def main():
                    "batch": "all queries in one hipGraph-captured plan (projection, probe select, fused search)",
                    "launch": {"wg_per_cu": wg, "table_log2": tl, "tile": "automatic (most buckets that fit)" if tl == 0
                               else "2^table_log2 slots", "tile_log2_floor": tile_log2, "lds_bytes": tile_lds,
-                              "proj_waves_per_wg": plan.proj_waves()}},
+                              "proj_waves_per_wg": plan.proj_waves(),
+                              "autotune": [{"wg_per_cu": a, "table_log2": b, "proj_waves": c_, "median_ms": round(t, 3)}
+                                           for a, b, c_, t in plan.tuning()]}},
         "recall_at_10": round(rec10, 4) if rec10 is not None else None,
         "recall_at_k": round(rec_k, 4),
         "plain_qps": round(plain_qps, 1) if plain_qps else None,

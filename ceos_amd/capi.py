@@ -54,6 +54,7 @@ def lib():
             "ceos_plan_run": (i32, [vp, i32, vp]),
             "ceos_plan_search": (i32, [vp, P, i32, i32, P, P, P, P, P]),
             "ceos_plan_config": (i32, [vp, P, P, P]),
+            "ceos_plan_tuning": (i32, [vp, i32, P, P, P]),
             "ceos_set_kernel_timing": (i32, [vp, i32]),
             "ceos_get_kernel_timing": (i32, [vp, P, P, P]),
             "ceos_read_bandwidth": (i32, [P, i64, P, i64, i32, vp, P]),

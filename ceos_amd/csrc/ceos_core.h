@@ -1023,6 +1023,12 @@ struct ceos_plan {
     } snap{};
     bool tuned = false;
     double tuned_ms = 0.0;
+    static constexpr int kTuneRuns = 3;
+    struct TuneRecord {
+        int wg, tl, pw;
+        double ms;
+    };
+    std::vector<TuneRecord> tune_log;  // every shape the tuner timed, in order
     int wg = 0, tl = 0, pw = 4;  // launch shape: workgroups per CU, tile log2, projection waves per workgroup
 
     // the last run: later runs on another stream wait for it; capture and destruction
@@ -1111,15 +1117,22 @@ struct ceos_plan {
         HIP_CHECK(hipEventCreate(&e1));
         double best = 1e30;
         Shape pick = cands[0];
+        tune_log.clear();
+        // each shape: one warm-up launch, then the median of kTuneRuns timed launches
         auto time_shape = [&](const Shape& cd) {
             capture(cd.wg, cd.tl, cd.pw);
             launch(nq, st);  // warm
-            HIP_CHECK(hipEventRecord(e0, st));
-            launch(nq, st);
-            HIP_CHECK(hipEventRecord(e1, st));
-            HIP_CHECK(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            float t[kTuneRuns];
+            for (int i = 0; i < kTuneRuns; ++i) {
+                HIP_CHECK(hipEventRecord(e0, st));
+                launch(nq, st);
+                HIP_CHECK(hipEventRecord(e1, st));
+                HIP_CHECK(hipEventSynchronize(e1));
+                HIP_CHECK(hipEventElapsedTime(&t[i], e0, e1));
+            }
+            std::sort(t, t + kTuneRuns);
+            const double ms = t[kTuneRuns / 2];
+            tune_log.push_back({cd.wg, cd.tl, cd.pw, ms});
             if (ms < best) {
                 best = ms;
                 pick = cd;

@@ -336,6 +336,14 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int ln) {
     return incl - v;
 }
 
+// Digit width of radix_select: 8 bits (256 bins); 4 bits can be built for comparison
+// with -DCEOS_RADIX_BITS=4.
+#ifndef CEOS_RADIX_BITS
+#define CEOS_RADIX_BITS 8
+#endif
+constexpr int kRadixBits = CEOS_RADIX_BITS;
+static_assert(kRadixBits == 4 || kRadixBits == 8, "radix digits of 4 or 8 bits");
+
 struct SelectShared {
     uint32_t hist[256];
     uint32_t bin, need, cnt, pad_;
@@ -360,11 +368,13 @@ __device__ __forceinline__ void radix_select(int n, int K, GetKey get, SelectSha
         shift_out = 63;
         return;
     }
+    constexpr int NB = 1 << kRadixBits;        // bins
+    constexpr int PB = (NB + kWave - 1) / kWave;  // bins per lane in the boundary scan
     uint64_t prefix = 0, mask = 0;
     uint32_t need = (uint32_t)K;
-    int shift = 56;
+    int shift = 64 - kRadixBits;
     for (;;) {
-        for (int b = tid; b < 256; b += NT) sh.hist[b] = 0;
+        for (int b = tid; b < NB; b += NT) sh.hist[b] = 0;
         __syncthreads();
         uint32_t run_bin = 0xffffffffu, run_cnt = 0;
         for (int i0 = tid; i0 < n; i0 += UNR * NT) {
@@ -381,7 +391,7 @@ __device__ __forceinline__ void radix_select(int n, int K, GetKey get, SelectSha
             for (int u = 0; u < UNR; ++u) {
                 const int i = i0 + u * NT;
                 if (i < n && (k[u] & mask) == prefix && (!kSkipZero || k[u] != 0ull)) {
-                    const uint32_t bn = (uint32_t)(k[u] >> shift) & 255u;
+                    const uint32_t bn = (uint32_t)(k[u] >> shift) & (uint32_t)(NB - 1);
                     if (bn != run_bin) {
                         if (run_cnt) atomicAdd(&sh.hist[run_bin], run_cnt);
                         run_bin = bn;
@@ -395,15 +405,19 @@ __device__ __forceinline__ void radix_select(int n, int K, GetKey get, SelectSha
         __syncthreads();
         if (tid < 64) {
             const int ln = tid;
-            uint32_t c[4], tot = 0;
+            uint32_t c[PB], tot = 0;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) { c[j] = sh.hist[255 - 4 * ln - j]; tot += c[j]; }
+            for (int j = 0; j < PB; ++j) {
+                const int b = NB - 1 - PB * ln - j;
+                c[j] = b >= 0 ? sh.hist[b] : 0u;
+                tot += c[j];
+            }
             uint32_t ex = wave_excl_scan(tot, ln);
             if (ex < need && need <= ex + tot) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < PB; ++j) {
                     if (need <= ex + c[j]) {
-                        sh.bin = 255 - 4 * ln - j;
+                        sh.bin = NB - 1 - PB * ln - j;
                         sh.need = need - ex;
                         sh.cnt = c[j];
                         break;
@@ -416,10 +430,10 @@ __device__ __forceinline__ void radix_select(int n, int K, GetKey get, SelectSha
         const uint32_t bin = sh.bin, nneed = sh.need, cnt = sh.cnt;
         __syncthreads();
         prefix |= (uint64_t)bin << shift;
-        mask |= (uint64_t)0xff << shift;
+        mask |= (uint64_t)(NB - 1) << shift;
         need = nneed;
         if (cnt == need || shift <= min_shift) break;
-        shift -= 8;
+        shift -= kRadixBits;
     }
     prefix_out = prefix;
     shift_out = shift;

@@ -110,6 +110,17 @@ class PyPlan {
         check(ceos_plan_proj_waves(p_, &pw));
         return pw;
     }
+    // [(wg_per_cu, table_log2, proj_waves, median ms), ...] of the autotuner
+    py::list tuning() const {
+        int cnt = 0;
+        check(ceos_plan_tuning(p_, 0, nullptr, nullptr, &cnt));
+        std::vector<int32_t> sh(3 * (size_t)cnt);
+        std::vector<double> ms((size_t)cnt);
+        if (cnt > 0) check(ceos_plan_tuning(p_, cnt, sh.data(), ms.data(), &cnt));
+        py::list out;
+        for (int i = 0; i < cnt; ++i) out.append(py::make_tuple(sh[3 * i], sh[3 * i + 1], sh[3 * i + 2], ms[i]));
+        return out;
+    }
 
    private:
     ceos_plan* p_;
@@ -746,7 +757,8 @@ PYBIND11_MODULE(CEOs, m) {
         .def("run", &PyPlan::run, py::arg("nq"), py::arg("stream") = 0)
         .def("buffers", &PyPlan::buffers)
         .def("config", &PyPlan::config)
-        .def("proj_waves", &PyPlan::proj_waves);
+        .def("proj_waves", &PyPlan::proj_waves)
+        .def("tuning", &PyPlan::tuning);
 
     py::class_<PyServer>(m, "Server")
         .def(py::init([](PyCEOs& ix, int max_batch, int k_max, int s_max, int c_max, int inflight, int64_t hbm_budget,

@@ -942,6 +942,20 @@ int ceos_plan_config(const ceos_plan* p, int* wg_per_cu, int* table_log2, double
     });
 }
 
+int ceos_plan_tuning(const ceos_plan* p, int cap, int32_t* shapes, double* ms, int* count) {
+    return guarded([&] {
+        if (!p || !count || cap < 0 || (cap > 0 && (!shapes || !ms))) throw ArgError("null argument");
+        const int nrec = (int)p->tune_log.size();
+        for (int i = 0; i < std::min(cap, nrec); ++i) {
+            shapes[3 * i] = p->tune_log[i].wg;
+            shapes[3 * i + 1] = p->tune_log[i].tl;
+            shapes[3 * i + 2] = p->tune_log[i].pw;
+            ms[i] = p->tune_log[i].ms;
+        }
+        *count = nrec;
+    });
+}
+
 int ceos_plan_proj_waves(const ceos_plan* p, int* waves_per_wg) {
     return guarded([&] {
         if (!p || !waves_per_wg) throw ArgError("null argument");

@@ -244,6 +244,11 @@ int ceos_plan_search(ceos_plan* p, const float* Q, int nq, int d, const int32_t*
 int ceos_plan_config(const ceos_plan* p, int* wg_per_cu, int* table_log2, double* tuned_ms);
 /* The query projection's waves per workgroup in use (2, 4 or 8; tuned with the rest). */
 int ceos_plan_proj_waves(const ceos_plan* p, int* waves_per_wg);
+/* The autotuner's record: for each launch shape it timed, in order, (workgroups per CU,
+ * table_log2 (0 = automatic tile), projection waves per workgroup) and the median kernel
+ * time in ms of 3 timed launches after a warm-up. Writes at most `cap` entries and sets
+ * *count to the number recorded (0 if untuned). */
+int ceos_plan_tuning(const ceos_plan* p, int cap, int32_t* shapes, double* ms, int* count);
 
 /* -------------------------------- serving --------------------------------
  * No reference counterpart: continuous batching over a resident index. Requests (one

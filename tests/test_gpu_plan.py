@@ -72,6 +72,13 @@ def test_plan_autotune_keeps_results(index):
     pi, pp = plan.search(Q, k, s, c)  # first run tunes
     wg, tl, ms = plan.config()
     assert ms > 0 and wg >= 1 and (tl == 0 or 8 <= tl <= 15)  # 0: the automatic tile
+    # the tuner's record: every shape it timed (median of 3 launches); the pick is the
+    # fastest of them, and both projection widths were tried with the picked shape
+    rec = plan.tuning()
+    assert len(rec) >= 4 and all(t > 0 for *_, t in rec)
+    assert ms == min(t for *_, t in rec)
+    assert {pw for *_, pw, _ in rec} >= {2, 4, 8}
+    assert (wg, tl, plan.proj_waves()) in {(a, b, c_) for a, b, c_, _ in rec}
     assert np.array_equal(pi, I0) and np.array_equal(pp, P0)
     pi, pp = plan.search(Q, k, s, c)
     assert np.array_equal(pi, I0) and np.array_equal(pp, P0)
