@@ -623,8 +623,7 @@ __host__ __device__ inline int quantize_lanes(int d) {
 }
 
 __global__ __launch_bounds__(256) void k_quantize_rows(const float* __restrict__ X, int64_t slot0, int64_t count,
-                                                       int64_t cap, int d, int8_t* __restrict__ Xq,
-                                                       float4* __restrict__ qsr) {
+                                                       int64_t cap, int d, int8_t* __restrict__ Xq) {
     const int ln = threadIdx.x % kWave;
     const int units = d >> 2;
     const int G = quantize_lanes(d);
@@ -634,7 +633,8 @@ __global__ __launch_bounds__(256) void k_quantize_rows(const float* __restrict__
     int64_t sl = slot0 + (ok ? j : 0);
     if (sl >= cap) sl -= cap;
     const float4* x4 = reinterpret_cast<const float4*>(X + sl * d);
-    uint32_t* c4 = reinterpret_cast<uint32_t*>(Xq + sl * d);
+    int8_t* xrow = Xq + sl * quant_stride(d);
+    uint32_t* c4 = reinterpret_cast<uint32_t*>(xrow);
     constexpr int KEEP = 4;
     float4 keep[KEEP];
     float mx = 0.0f;
@@ -699,18 +699,18 @@ __global__ __launch_bounds__(256) void k_quantize_rows(const float* __restrict__
             r1 = up((sqrt(e2) + g * sqrt(x2)) * (1.0 + 0x1p-30));
             r2 = up((double)s * sqrt((double)c2) * (1.0 + 0x1p-30));
         }
-        qsr[sl] = make_float4(s, r1, r2, 0.0f);
+        *reinterpret_cast<float4*>(xrow + quant_param_offset(d)) = make_float4(s, r1, r2, 0.0f);
     }
 }
 
 hipError_t launch_quantize_rows(const float* X, int64_t slot0, int64_t count, int64_t cap, int d, int8_t* Xq,
-                                float4* qsr, hipStream_t st) {
+                                hipStream_t st) {
     if (count <= 0) return hipSuccess;
     if (count > cap || slot0 < 0 || slot0 >= cap || d <= 0 || (d & 3) != 0) return hipErrorInvalidValue;
     const int64_t per_block = 4 * (kWave / quantize_lanes(d));
     const int64_t blocks = (count + per_block - 1) / per_block;
     if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_quantize_rows, dim3((unsigned)blocks), dim3(256), 0, st, X, slot0, count, cap, d, Xq, qsr);
+    hipLaunchKernelGGL(k_quantize_rows, dim3((unsigned)blocks), dim3(256), 0, st, X, slot0, count, cap, d, Xq);
     return hipGetLastError();
 }
 

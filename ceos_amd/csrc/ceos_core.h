@@ -230,11 +230,10 @@ struct Core {
     int64_t cap = 0, head = 0;
     uint32_t base = 0, live = 0;
     DBuf<uint32_t> dead;  // stream random-delete bitmap over ring slots
-    // rerank bounds: int8 row copies [cap][d] and per-slot (s, r1, r2, 0), kept in step
-    // with X for coCEOs and stream indexes (see k_quantize_rows); off: not kept, every
-    // candidate gets the exact rerank
+    // rerank bounds: int8 row copies [cap][quant_stride(d)] holding each row's codes and
+    // (s, r1, r2, 0), kept in step with X for coCEOs and stream indexes (see
+    // k_quantize_rows); off: not kept, every candidate gets the exact rerank
     DBuf<int8_t> Xq;
-    DBuf<float4> Xqs;
     bool rerank_bounds = true;
 
     // query scratch
@@ -426,7 +425,7 @@ struct Core {
         dcenter.release();
         center.clear();
         bkt.release(); bsize.release(); X.release(); signs.release(); dead.release();
-        Xq.release(); Xqs.release();
+        Xq.release();
         scr.release(); estP.release();
         bw_rows.release(); bw_sample.release(); bw_thr.release(); bw_base.release(); bw_cap.release();
         bw_count.release(); bw_ent.release(); bw_hcount.release(); bw_active.release();
@@ -669,7 +668,7 @@ struct Core {
         return scr.ensure(cfg, nq, DR());
     }
 
-    ceos::RowRing rows() const { return ceos::RowRing{X.p, cap, head, base, live, Xq.p, Xqs.p}; }
+    ceos::RowRing rows() const { return ceos::RowRing{X.p, cap, head, base, live, Xq.p}; }
 
     // Brings the rerank bounds up to date after rows were written at ring slots
     // [slot0, slot0 + count) mod cap. When the copies are missing or sized for another
@@ -677,16 +676,14 @@ struct Core {
     void quantize_rows(int64_t slot0, int64_t count, int64_t total, hipStream_t s_) {
         if (!rerank_bounds || d % 4 != 0 || cap <= 0) {
             Xq.release();
-            Xqs.release();
             return;
         }
-        if (Xq.n != (size_t)cap * d || Xqs.n != (size_t)cap) {
-            Xq.alloc((size_t)cap * d);
-            Xqs.alloc((size_t)cap);
+        if (Xq.n != (size_t)cap * ceos::quant_stride(d)) {
+            Xq.alloc((size_t)cap * ceos::quant_stride(d));
             slot0 = head;
             count = total;
         }
-        HIP_CHECK(ceos::launch_quantize_rows(X.p, slot0, count, cap, d, Xq.p, Xqs.p, s_));
+        HIP_CHECK(ceos::launch_quantize_rows(X.p, slot0, count, cap, d, Xq.p, s_));
     }
     // The rerank bounds of rows 0..n-1 made from src (a device copy of those rows, read
     // only) on the second stream, after the work queued on s_ so far: the memory-bound
@@ -694,17 +691,15 @@ struct Core {
     void quantize_rows_side(const float* src, int64_t n, hipStream_t s_) {
         if (!rerank_bounds || d % 4 != 0 || cap <= 0) {
             Xq.release();
-            Xqs.release();
             return;
         }
-        if (Xq.n != (size_t)cap * d || Xqs.n != (size_t)cap) {
-            Xq.alloc((size_t)cap * d);
-            Xqs.alloc((size_t)cap);
+        if (Xq.n != (size_t)cap * ceos::quant_stride(d)) {
+            Xq.alloc((size_t)cap * ceos::quant_stride(d));
         }
         hipStream_t a = aux();
         HIP_CHECK(hipEventRecord(fork_ev, s_));
         HIP_CHECK(hipStreamWaitEvent(a, fork_ev, 0));
-        HIP_CHECK(ceos::launch_quantize_rows(src, 0, n, n, d, Xq.p, Xqs.p, a));
+        HIP_CHECK(ceos::launch_quantize_rows(src, 0, n, n, d, Xq.p, a));
         HIP_CHECK(hipEventRecord(join_ev, a));
         aux_pending = true;
     }
@@ -713,7 +708,6 @@ struct Core {
         rerank_bounds = on;
         if (!on) {
             Xq.release();
-            Xqs.release();
         } else if (built && bkt.p && X.p) {
             quantize_rows(head, live, live, st());
             HIP_CHECK(hipStreamSynchronize(st()));
@@ -913,7 +907,6 @@ struct Core {
         dead.release();
         estP.release();
         Xq.release();
-        Xqs.release();
         D = fD; R = fR; m = fm; F = fF; L = fL;
         s = hdr[7]; mp = hdr[8]; c = hdr[9]; centering = hdr[10] != 0;
         seed = (int)(seed_v & 0x7fffffffu) == (int)seed_v ? (int)seed_v : -1;

@@ -28,11 +28,19 @@ struct RowRing {
     int64_t head;
     uint32_t base;   // absolute id of logical id 0
     uint32_t live;   // number of live rows
-    // int8 copies of the rows (same slots, row stride d) and per slot (s, r1, r2, 0):
-    // see k_quantize_rows. Null: every candidate gets the exact rerank.
+    // int8 copies of the rows (same slots, row stride quant_stride(d) bytes): the d codes,
+    // then at byte quant_param_offset(d) the row's (s, r1, r2, 0) as float4; see
+    // k_quantize_rows. Null: every candidate gets the exact rerank.
     const int8_t* Xq = nullptr;
-    const float4* qsr = nullptr;
 };
+
+// Layout of one int8 row copy: codes at [0, d), the float4 (s, r1, r2, 0) at the next
+// 16-byte boundary, the row padded to a multiple of 128 bytes (one L2 line). A row then
+// starts on a line and its bound terms share a line with its codes whenever the codes
+// leave 16 bytes free in their last line (d = 960: 1024 bytes, 8 lines, where codes
+// at stride d plus a separate 16-byte record touch 9).
+__host__ __device__ inline int quant_param_offset(int d) { return (d + 15) & ~15; }
+__host__ __device__ inline int quant_stride(int d) { return (quant_param_offset(d) + 16 + 127) & ~127; }
 
 // ---------------- build ----------------
 // Projects rows row(k) = first + k*stride (k < nrows) for all R repeats.
@@ -169,9 +177,9 @@ hipError_t launch_subtract_center(float* X, int64_t slot0, int64_t count, int64_
 hipError_t launch_clear_slot_bits(uint32_t* bits, int64_t slot0, int64_t count, int64_t cap, hipStream_t st);
 
 // Rerank bounds for ring slots (slot0 + j) mod cap, j < count: int8 row copies
-// Xq [cap][d] and per slot (s, r1, r2, 0); see k_quantize_rows.
+// Xq [cap][quant_stride(d)]: codes and (s, r1, r2, 0) per slot; see k_quantize_rows.
 hipError_t launch_quantize_rows(const float* X, int64_t slot0, int64_t count, int64_t cap, int d, int8_t* Xq,
-                                float4* qsr, hipStream_t st);
+                                hipStream_t st);
 
 // Streaming read of n floats (n % 4 == 0, x 16-byte aligned) by grid x 256 threads, one
 // partial sum per thread into out (grid * 256 floats).

@@ -72,7 +72,21 @@ template <int LG, int J>
 __device__ __forceinline__ void reduce_scatter16(float (&p)[16], int gl) {
     if constexpr (J < LG) {
         constexpr int O = (1 << LG) >> (J + 1);
-        if constexpr (J < (LG < 4 ? LG : 4)) {
+        if constexpr (J < (LG < 4 ? LG : 4) && (O == 32 || O == 16)) {
+            // v_permlane32_swap (lane bit 5) / v_permlane16_swap (lane bit 4) on the pair
+            // (p[k], p[HALF + k]) leaves a lane whose offset bit is clear with its own p[k]
+            // and the partner's p[k], and a lane whose bit is set with the partner's
+            // p[HALF + k] and its own: the same sums as keep + received, with no selects
+            constexpr int HALF = 16 >> (J + 1);
+#pragma unroll
+            for (int k = 0; k < HALF; ++k) {
+                const auto t = O == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(p[k]),
+                                                                          __float_as_uint(p[HALF + k]), false, false)
+                                       : __builtin_amdgcn_permlane16_swap(__float_as_uint(p[k]),
+                                                                          __float_as_uint(p[HALF + k]), false, false);
+                p[k] = __uint_as_float(t[0]) + __uint_as_float(t[1]);
+            }
+        } else if constexpr (J < (LG < 4 ? LG : 4)) {
             constexpr int HALF = 16 >> (J + 1);
             const bool up = (gl & O) != 0;
 #pragma unroll
@@ -983,8 +997,9 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 int64_t slot = (int64_t)idsrc[i] + rows.head;
                 return slot >= rows.cap ? slot - rows.cap : slot;
             };
+            const int qs = quant_stride(d), qpo = quant_param_offset(d);
             auto row_params = [&](int64_t slot) -> float4 {
-                const F4v t = ((GlobalF4)rows.qsr)[slot];
+                const F4v t = *(GlobalF4)(rows.Xq + slot * qs + qpo);
                 return make_float4(t.x, t.y, t.z, t.w);
             };
             auto emit = [&](int i, float a, float df) {
@@ -1054,8 +1069,12 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 // sums of NO = 16 >> min(LG, 4) consecutive rows starting at `rb`.
                 static_assert(kPruneU == 16, "the reduce-scatter below sums 16 rows per group");
                 const int lu = act ? gl : 0;
-                auto run = [&](auto lgc) __attribute__((always_inline)) {
+                // Row offsets: 32-bit when every slot's row lies within 4 GiB of the copies'
+                // base (loads then take the base in scalar registers), else 64-bit.
+                const bool off32 = (uint64_t)rows.cap * (uint64_t)qs <= 0xffffffffull;
+                auto run = [&](auto lgc, auto o32c) __attribute__((always_inline)) {
                     constexpr int LG = decltype(lgc)::value;
+                    constexpr bool O32 = decltype(o32c)::value;
                     constexpr int GS = 1 << LG;
                     constexpr int H = LG < 4 ? LG : 4;
                     constexpr int NO = U >> H;
@@ -1065,28 +1084,51 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                         if ((gl >> (LG - 1 - j)) & 1) rb += (U / 2) >> j;
                     // emitting lanes: one per row copy
                     const bool emitter = (gl & ((GS >> H) - 1)) == 0;
+                    const uint32_t head32 = (uint32_t)rows.head, cap32 = (uint32_t)rows.cap;
+                    const int step = kSWaves * groups;
                     for (int base = 0; base < ce; base += per_iter) {
-                        uint32_t id[U];
-#pragma unroll
-                        for (int u = 0; u < U; ++u)
-                            id[u] = idsrc[min(base + (u * kSWaves + wave) * groups + sub, ce - 1)];
-                        uint32_t pid[NO];
-#pragma unroll
-                        for (int t = 0; t < NO; ++t)
-                            pid[t] = idsrc[min(base + ((rb + t) * kSWaves + wave) * groups + sub, ce - 1)];
+                        const int i0 = base + wave * groups + sub;
                         U4v w[U];
-#pragma unroll
-                        for (int u = 0; u < U; ++u) {
-                            int64_t slot = (int64_t)id[u] + rows.head;
-                            if (slot >= rows.cap) slot -= rows.cap;
-                            w[u] = ((GlobalU4)(rows.Xq + slot * d))[lu];
-                        }
                         float4 prm[NO];
+                        if constexpr (O32) {
+                            uint32_t off[U];
 #pragma unroll
-                        for (int t = 0; t < NO; ++t) {
-                            int64_t slot = (int64_t)pid[t] + rows.head;
-                            if (slot >= rows.cap) slot -= rows.cap;
-                            prm[t] = row_params(slot);
+                            for (int u = 0; u < U; ++u) {
+                                const uint32_t sl = idsrc[min(i0 + u * step, ce - 1)] + head32;
+                                off[u] = min(sl, sl - cap32) * (uint32_t)qs;  // ring wrap: slot < 2 cap
+                            }
+                            uint32_t poff[NO];
+#pragma unroll
+                            for (int t = 0; t < NO; ++t) {
+                                const uint32_t sl = idsrc[min(i0 + (rb + t) * step, ce - 1)] + head32;
+                                poff[t] = min(sl, sl - cap32) * (uint32_t)qs + (uint32_t)qpo;
+                            }
+#pragma unroll
+                            for (int u = 0; u < U; ++u) w[u] = *(GlobalU4)(rows.Xq + (off[u] + 16u * (uint32_t)lu));
+#pragma unroll
+                            for (int t = 0; t < NO; ++t) {
+                                const F4v x = *(GlobalF4)(rows.Xq + poff[t]);
+                                prm[t] = make_float4(x.x, x.y, x.z, x.w);
+                            }
+                        } else {
+                            uint32_t id[U];
+#pragma unroll
+                            for (int u = 0; u < U; ++u) id[u] = idsrc[min(i0 + u * step, ce - 1)];
+                            uint32_t pid[NO];
+#pragma unroll
+                            for (int t = 0; t < NO; ++t) pid[t] = idsrc[min(i0 + (rb + t) * step, ce - 1)];
+#pragma unroll
+                            for (int u = 0; u < U; ++u) {
+                                int64_t slot = (int64_t)id[u] + rows.head;
+                                if (slot >= rows.cap) slot -= rows.cap;
+                                w[u] = ((GlobalU4)(rows.Xq + slot * qs))[lu];
+                            }
+#pragma unroll
+                            for (int t = 0; t < NO; ++t) {
+                                int64_t slot = (int64_t)pid[t] + rows.head;
+                                if (slot >= rows.cap) slot -= rows.cap;
+                                prm[t] = row_params(slot);
+                            }
                         }
                         float p[U];
 #pragma unroll
@@ -1105,19 +1147,25 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                         reduce_scatter16<LG, 0>(p, gl);
 #pragma unroll
                         for (int t = 0; t < NO; ++t) {
-                            const int i = base + ((rb + t) * kSWaves + wave) * groups + sub;
+                            const int i = i0 + (rb + t) * step;
                             if (emitter && i < ce)
                                 emit(i, prm[t].x * p[t], fmaf(qn, prm[t].y, beta * prm[t].z) * (1.0f + 0x1p-20f));
                         }
                     }
                 };
-                switch (G) {
-                    case 64: run(std::integral_constant<int, 6>{}); break;
-                    case 32: run(std::integral_constant<int, 5>{}); break;
-                    case 16: run(std::integral_constant<int, 4>{}); break;
-                    case 8: run(std::integral_constant<int, 3>{}); break;
-                    default: run(std::integral_constant<int, 2>{}); break;
-                }
+                auto run_g = [&](auto o32c) __attribute__((always_inline)) {
+                    switch (G) {
+                        case 64: run(std::integral_constant<int, 6>{}, o32c); break;
+                        case 32: run(std::integral_constant<int, 5>{}, o32c); break;
+                        case 16: run(std::integral_constant<int, 4>{}, o32c); break;
+                        case 8: run(std::integral_constant<int, 3>{}, o32c); break;
+                        default: run(std::integral_constant<int, 2>{}, o32c); break;
+                    }
+                };
+                if (off32)
+                    run_g(std::true_type{});
+                else
+                    run_g(std::false_type{});
             } else {
                 constexpr int UF = kPruneU / 2;
                 const int per_iter_f = kSWaves * groups * UF;
@@ -1134,7 +1182,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                         prm[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         if (i < ce) {
                             const int64_t slot = slot_of(i);
-                            rp[u] = rows.Xq + slot * d;
+                            rp[u] = rows.Xq + slot * qs;
                             prm[u] = row_params(slot);
                         }
                     }
