@@ -63,43 +63,52 @@ constexpr int kStashList = kSearchStashList;   // pending stash entries per pass
 constexpr int kPruneU = 16;                    // bound pass: int8 rows in flight per lane group
 constexpr int kSurvivorMax = kSelBins;         // bound pass: exact-rerank list capacity (hist region)
 
-// Step J of the reduce-scatter of 16 per-lane values over groups of 2^LG lanes: the
-// first min(LG, 4) steps exchange half of the values still carried (the lane whose
-// offset bit is set keeps the upper half), the rest are plain butterflies. Afterwards
-// p[t], t < 16 >> min(LG, 4), is the group sum of value rb + t, where rb sums 8 >> j
-// over the steps j whose offset bit is set in the lane's group index.
+// Step J of the reduce-scatter of 16 per-lane (h, l) int32 pairs over groups of 2^LG
+// lanes: the first min(LG, 4) steps exchange half of the pairs still carried (the lane
+// whose offset bit is set keeps the upper half; across lane bits 5 and 4 a
+// v_permlane32/16_swap of the pair (v[k], v[HALF + k]) leaves each lane its own half and
+// the partner's copy of it, with no selects), the rest are plain butterflies. Afterwards
+// h[t], l[t] (t < 16 >> min(LG, 4)) are the group totals of pair rb + t, where rb sums
+// 8 >> j over the steps j whose offset bit is set in the lane's group index. Integer
+// sums are exact, so the order of the exchanges does not matter.
 template <int LG, int J>
-__device__ __forceinline__ void reduce_scatter16(float (&p)[16], int gl) {
+__device__ __forceinline__ void reduce_scatter16i(int (&h)[16], int (&l)[16], int gl) {
     if constexpr (J < LG) {
         constexpr int O = (1 << LG) >> (J + 1);
         if constexpr (J < (LG < 4 ? LG : 4) && (O == 32 || O == 16)) {
-            // v_permlane32_swap (lane bit 5) / v_permlane16_swap (lane bit 4) on the pair
-            // (p[k], p[HALF + k]) leaves a lane whose offset bit is clear with its own p[k]
-            // and the partner's p[k], and a lane whose bit is set with the partner's
-            // p[HALF + k] and its own: the same sums as keep + received, with no selects
             constexpr int HALF = 16 >> (J + 1);
 #pragma unroll
             for (int k = 0; k < HALF; ++k) {
-                const auto t = O == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(p[k]),
-                                                                          __float_as_uint(p[HALF + k]), false, false)
-                                       : __builtin_amdgcn_permlane16_swap(__float_as_uint(p[k]),
-                                                                          __float_as_uint(p[HALF + k]), false, false);
-                p[k] = __uint_as_float(t[0]) + __uint_as_float(t[1]);
+                const auto a = O == 32 ? __builtin_amdgcn_permlane32_swap((uint32_t)h[k], (uint32_t)h[HALF + k], false, false)
+                                       : __builtin_amdgcn_permlane16_swap((uint32_t)h[k], (uint32_t)h[HALF + k], false, false);
+                const auto b = O == 32 ? __builtin_amdgcn_permlane32_swap((uint32_t)l[k], (uint32_t)l[HALF + k], false, false)
+                                       : __builtin_amdgcn_permlane16_swap((uint32_t)l[k], (uint32_t)l[HALF + k], false, false);
+                h[k] = (int)(a[0] + a[1]);
+                l[k] = (int)(b[0] + b[1]);
             }
         } else if constexpr (J < (LG < 4 ? LG : 4)) {
             constexpr int HALF = 16 >> (J + 1);
             const bool up = (gl & O) != 0;
 #pragma unroll
             for (int k = 0; k < HALF; ++k) {
-                const float keep = up ? p[HALF + k] : p[k];
-                const float send = up ? p[k] : p[HALF + k];
-                p[k] = keep + __shfl_xor(send, O, kWave);
+                const int kh = up ? h[HALF + k] : h[k], sh = up ? h[k] : h[HALF + k];
+                const int kl = up ? l[HALF + k] : l[k], sl = up ? l[k] : l[HALF + k];
+                h[k] = kh + __shfl_xor(sh, O, kWave);
+                l[k] = kl + __shfl_xor(sl, O, kWave);
             }
         } else {
-            p[0] += __shfl_xor(p[0], O, kWave);
+            h[0] += __shfl_xor(h[0], O, kWave);
+            l[0] += __shfl_xor(l[0], O, kWave);
         }
-        reduce_scatter16<LG, J + 1>(p, gl);
+        reduce_scatter16i<LG, J + 1>(h, l, gl);
     }
+}
+
+// v_dot4_i32_i8 with a zero accumulator (the builtin with 0 becomes a move and v_dot4c).
+__device__ __forceinline__ int sdot4_first(uint32_t a, uint32_t b) {
+    int r;
+    asm("v_dot4_i32_i8 %0, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b));
+    return r;
 }
 
 // FP32 sum of four int8 lanes of w (byte j = element j) times q, added to acc.
@@ -965,9 +974,10 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         // rounding of any FP32 sum of depth below d + 64).
         //  * Integer path (d % 16 == 0, 64 <= d <= 1024; one 16-byte unit per lane): the
         //    query is coded as q^ = t1 b1 + t2 b2 (int8 codes of q and of its residual);
-        //    a = s * fl(sum over lanes of t1 I1 + t2 I2), I = exact int32 dot products of
-        //    the codes with c (v_dot4). |v - a| <= |q| r1 + (|q - q^| + 12u (t1|b1| +
-        //    t2|b2|)) r2, u = 2^-24.
+        //    I1, I2 = exact int32 dot products of the codes with c (v_dot4 per lane, then an
+        //    integer reduce-scatter; |I| <= 127^2 d < 2^24, so each converts exactly) and
+        //    a = s * fl(t1 I1 + t2 I2). |v - a| <= |q| r1 + (|q - q^| + 12u (t1|b1| +
+        //    t2|b2|)) r2, u = 2^-24 (the two roundings of t1 I1 + t2 I2 take 2u of the 12u).
         //  * Float path (other d): a = s * fl(q . c), |v - a| <= |q| (r1 + g r2).
         // The margin added to the bound covers the rounding of the bound, lo and hi.
         auto bound_pass = [&](const uint32_t* idsrc, uint64_t* keydst) __attribute__((always_inline)) {
@@ -1069,9 +1079,10 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 // sums of NO = 16 >> min(LG, 4) consecutive rows starting at `rb`.
                 static_assert(kPruneU == 16, "the reduce-scatter below sums 16 rows per group");
                 const int lu = act ? gl : 0;
-                // Row offsets: 32-bit when every slot's row lies within 4 GiB of the copies'
-                // base (loads then take the base in scalar registers), else 64-bit.
-                const bool off32 = (uint64_t)rows.cap * (uint64_t)qs <= 0xffffffffull;
+                // Row offsets: 32-bit, with a 24-bit multiply, when the ring has at most 2^24
+                // slots and every row lies within 4 GiB of the copies' base (loads then take
+                // the base in scalar registers); else 64-bit.
+                const bool off32 = rows.cap <= (1 << 24) && (uint64_t)rows.cap * (uint64_t)qs <= 0xffffffffull;
                 auto run = [&](auto lgc, auto o32c) __attribute__((always_inline)) {
                     constexpr int LG = decltype(lgc)::value;
                     constexpr bool O32 = decltype(o32c)::value;
@@ -1095,13 +1106,13 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
 #pragma unroll
                             for (int u = 0; u < U; ++u) {
                                 const uint32_t sl = idsrc[min(i0 + u * step, ce - 1)] + head32;
-                                off[u] = min(sl, sl - cap32) * (uint32_t)qs;  // ring wrap: slot < 2 cap
+                                off[u] = __umul24(min(sl, sl - cap32), (uint32_t)qs);  // ring wrap: slot < 2 cap
                             }
                             uint32_t poff[NO];
 #pragma unroll
                             for (int t = 0; t < NO; ++t) {
                                 const uint32_t sl = idsrc[min(i0 + (rb + t) * step, ce - 1)] + head32;
-                                poff[t] = min(sl, sl - cap32) * (uint32_t)qs + (uint32_t)qpo;
+                                poff[t] = __umul24(min(sl, sl - cap32), (uint32_t)qs) + (uint32_t)qpo;
                             }
 #pragma unroll
                             for (int u = 0; u < U; ++u) w[u] = *(GlobalU4)(rows.Xq + (off[u] + 16u * (uint32_t)lu));
@@ -1130,26 +1141,27 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                                 prm[t] = row_params(slot);
                             }
                         }
-                        float p[U];
+                        int hs[U], ls[U];
 #pragma unroll
                         for (int u = 0; u < U; ++u) {
-                            int h = 0, l = 0;
-                            h = __builtin_amdgcn_sdot4((int)w[u].x, (int)w1[0], h, false);
-                            l = __builtin_amdgcn_sdot4((int)w[u].x, (int)w2[0], l, false);
+                            int h = sdot4_first(w[u].x, w1[0]);
+                            int l = sdot4_first(w[u].x, w2[0]);
                             h = __builtin_amdgcn_sdot4((int)w[u].y, (int)w1[1], h, false);
                             l = __builtin_amdgcn_sdot4((int)w[u].y, (int)w2[1], l, false);
                             h = __builtin_amdgcn_sdot4((int)w[u].z, (int)w1[2], h, false);
                             l = __builtin_amdgcn_sdot4((int)w[u].z, (int)w2[2], l, false);
                             h = __builtin_amdgcn_sdot4((int)w[u].w, (int)w1[3], h, false);
                             l = __builtin_amdgcn_sdot4((int)w[u].w, (int)w2[3], l, false);
-                            p[u] = fmaf(t1, (float)h, t2 * (float)l);
+                            hs[u] = h;
+                            ls[u] = l;
                         }
-                        reduce_scatter16<LG, 0>(p, gl);
+                        reduce_scatter16i<LG, 0>(hs, ls, gl);
 #pragma unroll
                         for (int t = 0; t < NO; ++t) {
                             const int i = i0 + (rb + t) * step;
                             if (emitter && i < ce)
-                                emit(i, prm[t].x * p[t], fmaf(qn, prm[t].y, beta * prm[t].z) * (1.0f + 0x1p-20f));
+                                emit(i, prm[t].x * fmaf(t1, (float)hs[t], t2 * (float)ls[t]),
+                                     fmaf(qn, prm[t].y, beta * prm[t].z) * (1.0f + 0x1p-20f));
                         }
                     }
                 };
