@@ -240,8 +240,8 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=-1.0,
                     help="offered rate of the served run (default: 60%% of the batched rate; 0 = skip)")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
-    ap.add_argument("--serve-inflight", type=int, default=2, help="batch slots of the served run")
-    ap.add_argument("--serve-warmup-seconds", type=float, default=0.2,
+    ap.add_argument("--serve-inflight", type=int, default=4, help="batch slots of the served run")
+    ap.add_argument("--serve-warmup-seconds", type=float, default=0.5,
                     help="untimed served run at the same rate before the measured one")
     ap.add_argument("--serve-max-batch", type=int, default=4096, help="queries per batch slot of the served run")
     ap.add_argument("--no-autotune", action="store_true")
@@ -440,7 +440,7 @@ def main():
             f"p99 {np.percentile(lat, 99):.3f} ms")
         capi.check(L.ceos_set_query_param(h, w.s, w.m, w.c))
 
-    # ---- served: the continuous-batching server (two batch slots of up to 4096 queries)
+    # ---- served: the continuous-batching server (four batch slots of up to 4096 queries)
     # over the same index, open-loop Poisson arrivals of the workload's queries with their
     # per-query parameters; latency from arrival to results in host memory
     served = None

@@ -39,7 +39,9 @@ def main():
     ap.add_argument("--seconds", type=float, default=2.0, help="arrival window per rate")
     ap.add_argument("--n", type=int, default=0, help="override number of points (testing only)")
     ap.add_argument("--max-batch", type=int, default=4096)
-    ap.add_argument("--inflight", type=int, default=2)
+    ap.add_argument("--inflight", type=int, default=4, help="batch slots")
+    ap.add_argument("--warmup-seconds", type=float, default=0.5,
+                    help="untimed arrivals at the first rate before the measured rates")
     ap.add_argument("--check", type=int, default=64, help="served results checked against alone")
     args = ap.parse_args()
 
@@ -75,7 +77,21 @@ def main():
     upd_rng = np.random.default_rng(4)
     next_row = [0]
 
-    for rate in [int(r) for r in args.rates.split(",")]:
+    rates = [int(r) for r in args.rates.split(",")]
+    if args.warmup_seconds > 0 and rates:  # the server and the GPU at load first, untimed
+        nw = max(1, int(rates[0] * args.warmup_seconds))
+        Qw = make_queries(w, nw, seed=7)
+        if uneven:
+            kw, sw, cw = make_query_params(w, nw, seed=8)
+        else:
+            kw = np.full(nw, min(10, k_max), np.int32)
+            sw = np.full(nw, w.s, np.int32)
+            cw = np.full(nw, min(w.c, c_max), np.int32)
+        srv.submit(Qw, kw, sw, cw, np.cumsum(rng.exponential(1.0 / rates[0], nw)))
+        srv.drain()
+        del Qw
+
+    for rate in rates:
         nreq = int(rate * args.seconds)
         Q = make_queries(w, nreq, seed=1 + rate)
         if stream:
