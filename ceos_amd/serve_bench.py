@@ -39,7 +39,8 @@ def main():
     ap.add_argument("--seconds", type=float, default=2.0, help="arrival window per rate")
     ap.add_argument("--n", type=int, default=0, help="override number of points (testing only)")
     ap.add_argument("--max-batch", type=int, default=4096)
-    ap.add_argument("--inflight", type=int, default=4, help="batch slots")
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="batch slots (more slots overlap large batches' tails; fewer keep small batches fast)")
     ap.add_argument("--warmup-seconds", type=float, default=0.5,
                     help="untimed arrivals at the first rate before the measured rates")
     ap.add_argument("--check", type=int, default=64, help="served results checked against alone")

@@ -159,8 +159,9 @@ int ceos_stream_set_build_budget(ceos_stream* h, int64_t entries);
 int ceos_get_build_rescans(const ceos_index* h, int* under, int* over, int* passes);
 
 /* Rerank bounds (no reference counterpart; on by default). While on, a coCEOs or
- * streamCEOs index keeps an int8 copy of every row (n * d bytes, a quarter of the FP32
- * rows) with a per-row scale and error bound, built with the index and kept in step by
+ * streamCEOs index keeps an int8 copy of every row with its scale and error bound, each
+ * row padded to a multiple of 128 bytes (about a quarter of the FP32 rows), built with
+ * the index and kept in step by
  * stream updates. A search whose k is at most 256 and at most a quarter of its
  * candidates then bounds every candidate's inner product from the int8 row and gives
  * the exact FP32 rerank only to candidates that can reach the top-k: same results,
