@@ -737,6 +737,38 @@ hipError_t launch_stream_read(const float* x, int64_t n, float* out, int grid, h
     return hipGetLastError();
 }
 
+// Segment blockIdx.y; 16-byte words when the segment allows, else 4-byte words.
+__global__ __launch_bounds__(256) void k_copy_segments(CopySegs c) {
+    const int g = blockIdx.y;
+    const uint32_t* __restrict__ src = c.src[g];
+    uint32_t* __restrict__ dst = c.dst[g];
+    const int64_t n = c.words[g];
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool v4 = (n & 3) == 0 && ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0;
+    if (v4) {
+        const uint4* s4 = reinterpret_cast<const uint4*>(src);
+        uint4* d4 = reinterpret_cast<uint4*>(dst);
+        for (int64_t i = t0; i < (n >> 2); i += step) d4[i] = s4[i];
+    } else {
+        for (int64_t i = t0; i < n; i += step) dst[i] = src[i];
+    }
+}
+
+hipError_t launch_copy_segments(const CopySegs& c, hipStream_t st) {
+    if (c.n <= 0) return hipSuccess;
+    if (c.n > 4) return hipErrorInvalidValue;
+    int64_t most = 0;
+    for (int g = 0; g < c.n; ++g) {
+        if (c.words[g] < 0 || (c.words[g] > 0 && (!c.src[g] || !c.dst[g]))) return hipErrorInvalidValue;
+        most = std::max(most, c.words[g]);
+    }
+    if (most == 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>(1024, (most + 4 * 256 - 1) / (4 * 256));
+    hipLaunchKernelGGL(k_copy_segments, dim3((unsigned)blocks, (unsigned)c.n), dim3(256), 0, st, c);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     const int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);

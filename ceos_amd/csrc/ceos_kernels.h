@@ -165,6 +165,17 @@ int est_search_lds_bytes(const IndexDims& dm, int s_max, int c_max, int k_max);
 
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 
+// Up to four independent copies of 4-byte words in one launch (for example from mapped
+// pinned host memory to device buffers, or back): a kernel on the stream instead of
+// copy-engine transfers. No destination range may overlap a source range.
+struct CopySegs {
+    const uint32_t* src[4];
+    uint32_t* dst[4];
+    int64_t words[4];
+    int n;
+};
+hipError_t launch_copy_segments(const CopySegs& c, hipStream_t st);
+
 // Centering: center[j] = FP32 of the FP64 mean of column j over rows [0, n) of X, summed
 // in a fixed order (partial: column_mean_blocks(n) x d doubles of workspace).
 int column_mean_blocks(int64_t n);

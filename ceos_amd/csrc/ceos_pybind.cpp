@@ -694,7 +694,7 @@ class PyServer {
         int64_t got = 0;
         check(ceos_server_get_batches(live(), b.data(), n, &got));
         n = std::min(n, got);
-        py::array_t<double> out({(py::ssize_t)n, (py::ssize_t)6});
+        py::array_t<double> out({(py::ssize_t)n, (py::ssize_t)11});
         auto m = out.mutable_unchecked<2>();
         for (int64_t i = 0; i < n; ++i) {
             m(i, 0) = b[i].launch_s;
@@ -703,6 +703,7 @@ class PyServer {
             m(i, 3) = b[i].oldest_wait_ms;
             m(i, 4) = b[i].queries;
             m(i, 5) = b[i].slot;
+            for (int j = 0; j < 5; ++j) m(i, 6 + j) = b[i].steps_ms[j];
         }
         return out;
     }

@@ -61,17 +61,28 @@ struct Update {
     std::vector<int32_t> ids;  // random delete
 };
 
-// Pinned host memory (hipHostMalloc).
+// Pinned host memory (hipHostMalloc), mapped into the device's address space: `dp` is the
+// address kernels use. Coherent, so a kernel reads what the host wrote before its launch
+// and the host reads what a kernel wrote once the kernel's completion event is seen.
 template <typename T>
 struct HBuf {
     T* p = nullptr;
+    T* dp = nullptr;
     void alloc(size_t n) {
-        if (hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(1, n) * sizeof(T), hipHostMallocDefault) !=
-            hipSuccess) {
+        if (hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(1, n) * sizeof(T),
+                          hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
             (void)hipGetLastError();
             p = nullptr;
             throw MemError("pinned host allocation failed");
         }
+        void* d = nullptr;
+        if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess || !d) {
+            (void)hipGetLastError();
+            (void)hipHostFree(p);
+            p = nullptr;
+            throw MemError("pinned host memory is not mapped for the device");
+        }
+        dp = static_cast<T*>(d);
     }
     ~HBuf() {
         if (p) (void)hipHostFree(p);
@@ -88,6 +99,43 @@ struct Slot {
     bool busy = false;
     Clock::time_point t_launch{};
     ceos_server_batch cur{};  // trace record of the batch in flight
+
+    // Enqueues a staged batch of nb requests on the slot's stream: the inputs move from the
+    // mapped staging into the plan's buffers by a kernel, the plan runs, the results move
+    // back by a kernel, and the completion event is recorded. Copy-engine transfers
+    // (hipMemcpyAsync) blocked the worker for several milliseconds at times. ts[1] and
+    // ts[2] (optional) get the times after the inputs' launch and after the plan's.
+    void enqueue(int nb, int d, int k_max, Clock::time_point* ts = nullptr) {
+        float* dQ;
+        int32_t *dk, *ds, *dc, *dI;
+        float* dP;
+        if (ceos_plan_buffers(plan, &dQ, &dk, &ds, &dc, &dI, &dP) != CEOS_OK) throw DevError(g_err);
+        ceos::CopySegs in{};
+        in.n = 4;
+        in.src[0] = reinterpret_cast<const uint32_t*>(hQ.dp);
+        in.dst[0] = reinterpret_cast<uint32_t*>(dQ);
+        in.words[0] = (int64_t)nb * d;
+        in.src[1] = reinterpret_cast<const uint32_t*>(hk.dp);
+        in.dst[1] = reinterpret_cast<uint32_t*>(dk);
+        in.src[2] = reinterpret_cast<const uint32_t*>(hs.dp);
+        in.dst[2] = reinterpret_cast<uint32_t*>(ds);
+        in.src[3] = reinterpret_cast<const uint32_t*>(hc.dp);
+        in.dst[3] = reinterpret_cast<uint32_t*>(dc);
+        in.words[1] = in.words[2] = in.words[3] = nb;
+        HIP_CHECK(ceos::launch_copy_segments(in, st));
+        if (ts) ts[1] = Clock::now();
+        if (ceos_plan_run(plan, nb, st) != CEOS_OK) throw DevError(g_err);
+        if (ts) ts[2] = Clock::now();
+        ceos::CopySegs out{};
+        out.n = 2;
+        out.src[0] = reinterpret_cast<const uint32_t*>(dI);
+        out.dst[0] = reinterpret_cast<uint32_t*>(hI.dp);
+        out.src[1] = reinterpret_cast<const uint32_t*>(dP);
+        out.dst[1] = reinterpret_cast<uint32_t*>(hP.dp);
+        out.words[0] = out.words[1] = (int64_t)nb * k_max;
+        HIP_CHECK(ceos::launch_copy_segments(out, st));
+        HIP_CHECK(hipEventRecord(ev, st));
+    }
 };
 
 }  // namespace
@@ -275,20 +323,11 @@ struct ceos_server {
         // of this batch can be collected before it completes, so the submissions stay alive
         for (int b = 0; b < nb; ++b)
             std::memcpy(sl.hQ.p + (size_t)b * d, taken_[b].f->Q.data() + (size_t)taken_[b].i * d, (size_t)d * 4);
+        Clock::time_point ts[5];
+        ts[0] = Clock::now();
         try {
-            float* dQ;
-            int32_t *dk, *ds, *dc, *dI;
-            float* dP;
-            HIP_CHECK(ceos_plan_buffers(sl.plan, &dQ, &dk, &ds, &dc, &dI, &dP) == CEOS_OK ? hipSuccess
-                                                                                            : hipErrorInvalidValue);
-            HIP_CHECK(hipMemcpyAsync(dQ, sl.hQ.p, (size_t)nb * d * 4, hipMemcpyHostToDevice, sl.st));
-            HIP_CHECK(hipMemcpyAsync(dk, sl.hk.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl.st));
-            HIP_CHECK(hipMemcpyAsync(ds, sl.hs.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl.st));
-            HIP_CHECK(hipMemcpyAsync(dc, sl.hc.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl.st));
-            if (ceos_plan_run(sl.plan, nb, sl.st) != CEOS_OK) throw DevError(g_err);
-            HIP_CHECK(hipMemcpyAsync(sl.hI.p, dI, (size_t)nb * cfg.k_max * 4, hipMemcpyDeviceToHost, sl.st));
-            HIP_CHECK(hipMemcpyAsync(sl.hP.p, dP, (size_t)nb * cfg.k_max * 4, hipMemcpyDeviceToHost, sl.st));
-            HIP_CHECK(hipEventRecord(sl.ev, sl.st));
+            sl.enqueue(nb, d, cfg.k_max, ts);
+            ts[3] = Clock::now();
         } catch (const std::exception& e) {
             lk.lock();
             sl.busy = false;
@@ -296,7 +335,10 @@ struct ceos_server {
             return;
         }
         lk.lock();
-        sl.cur.enqueue_ms = std::chrono::duration<double, std::milli>(Clock::now() - sl.t_launch).count();
+        ts[4] = Clock::now();
+        sl.cur.enqueue_ms = std::chrono::duration<double, std::milli>(ts[4] - sl.t_launch).count();
+        for (int i = 0; i < 5; ++i)
+            sl.cur.steps_ms[i] = std::chrono::duration<double, std::milli>(ts[i] - (i ? ts[i - 1] : sl.t_launch)).count();
     }
 
     void apply_update(Update& u, std::unique_lock<std::mutex>& lk) {
@@ -454,17 +496,7 @@ int make_server(Core& x, ceos_index* hi, ceos_stream* hs, const ceos_server_conf
                     sl->hs.p[i] = std::min(cfg->s_max, 16);
                     sl->hc.p[i] = std::min(cfg->c_max, 500);
                 }
-                float* dQ;
-                int32_t *dk, *ds, *dc, *dI;
-                float* dP;
-                if (ceos_plan_buffers(sl->plan, &dQ, &dk, &ds, &dc, &dI, &dP) != CEOS_OK) throw DevError(g_err);
-                HIP_CHECK(hipMemcpyAsync(dQ, sl->hQ.p, (size_t)nb * x.d * 4, hipMemcpyHostToDevice, sl->st));
-                HIP_CHECK(hipMemcpyAsync(dk, sl->hk.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl->st));
-                HIP_CHECK(hipMemcpyAsync(ds, sl->hs.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl->st));
-                HIP_CHECK(hipMemcpyAsync(dc, sl->hc.p, (size_t)nb * 4, hipMemcpyHostToDevice, sl->st));
-                if (ceos_plan_run(sl->plan, nb, sl->st) != CEOS_OK) throw DevError(g_err);
-                HIP_CHECK(hipMemcpyAsync(sl->hI.p, dI, (size_t)nb * cfg->k_max * 4, hipMemcpyDeviceToHost, sl->st));
-                HIP_CHECK(hipMemcpyAsync(sl->hP.p, dP, (size_t)nb * cfg->k_max * 4, hipMemcpyDeviceToHost, sl->st));
+                sl->enqueue(nb, x.d, cfg->k_max);
                 HIP_CHECK(hipStreamSynchronize(sl->st));
             }
             srv->slots.push_back(std::move(sl));

@@ -32,6 +32,19 @@ def log(msg):
     print(f"[serve] {msg}", file=sys.stderr, flush=True)
 
 
+def slowest(trace, n=3):
+    """The n batches that took longest from launch to results, from the server's batch trace
+    (launch time s, enqueue ms, done ms, oldest request's wait ms, queries, slot, and the
+    enqueue's steps in ms: rows to staging, inputs to device, plan launch, results copy and
+    event, server lock)."""
+    if len(trace) == 0:
+        return []
+    return [{"launch_s": round(float(b[0]), 4), "enqueue_ms": round(float(b[1]), 3), "done_ms": round(float(b[2]), 3),
+             "oldest_wait_ms": round(float(b[3]), 3), "queries": int(b[4]),
+             "enqueue_steps_ms": [round(float(x), 3) for x in b[6:11]]}
+            for b in trace[np.argsort(-trace[:, 2])[:n]]]
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--workload", default="serve_10m", choices=["serve_10m", "stream_5m", "gist_like"])
@@ -157,7 +170,14 @@ def main():
                "insert_10k_mean_ms": round(st["insert_mean_ms"], 3) if stream else None,
                "insert_10k_max_ms": round(st["insert_max_ms"], 3) if stream else None,
                "delete_mean_ms": round(st["delete_mean_ms"], 3) if stream else None,
-               "delete_max_ms": round(st["delete_max_ms"], 3) if stream else None}
+               "delete_max_ms": round(st["delete_max_ms"], 3) if stream else None,
+               # where the time goes: a request's wait before its batch is launched, and a
+               # batch's time from launch to results in host memory
+               "wait_ms": {"p50": round(st["wait_p50_ms"], 3), "p99": round(st["wait_p99_ms"], 3),
+                           "max": round(st["wait_max_ms"], 3)},
+               "batch_service_ms": {"p50": round(st["service_p50_ms"], 3), "p99": round(st["service_p99_ms"], 3),
+                                    "max": round(st["service_max_ms"], 3)},
+               "slowest_batches": slowest(srv.batches())}
         log(json.dumps(out))
         print(json.dumps(out), flush=True)
     srv.close()

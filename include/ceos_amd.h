@@ -316,6 +316,9 @@ typedef struct {
     double done_ms;         /* launch -> results in host memory, when the worker reaped them */
     double oldest_wait_ms;  /* arrival of the batch's oldest request -> launch */
     int32_t queries, slot;
+    /* the enqueue's steps, ms: rows to the pinned staging, inputs to the device, the plan's
+     * launch, the results' copy and the completion event, the server lock taken again */
+    double steps_ms[5];
 } ceos_server_batch;
 int ceos_server_get_batches(ceos_server* s, ceos_server_batch* out, int64_t max, int64_t* count);
 int ceos_server_reset_stats(ceos_server* s);

@@ -152,11 +152,13 @@ def test_serve_batch_trace(CEOs, index):
     srv.drain()
     B = srv.batches()
     st = srv.stats()
-    assert B.shape == (st["batches"], 6)
+    assert B.shape == (st["batches"], 11)
     assert int(B[:, 4].sum()) == n and (B[:, 4] <= 64).all()
     assert np.all(B[:, 0] >= 0) and B[:, 0].max() < st["elapsed_s"] + 1.0   # launched within the run
     assert np.all(B[:, 2] >= B[:, 1]) and np.all(B[:, 3] >= 0)
     assert set(B[:, 5].astype(int).tolist()) <= {0, 1}
+    # the enqueue's steps add up to the enqueue time
+    assert np.all(B[:, 6:] >= 0) and np.allclose(B[:, 6:].sum(1), B[:, 1], rtol=1e-6, atol=1e-6)
     srv.reset_stats()
-    assert srv.batches().shape == (0, 6)
+    assert srv.batches().shape == (0, 11)
     srv.close()
