@@ -543,7 +543,8 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             if (tid == 0) { misc[2] = 0; misc[4] = 0; }
             bool overflow = false;
             for (uint32_t pass = 0; pass < P && !overflow; ++pass) {
-                for (int i = tid; i < T; i += kSNT) keys[i] = kEmptyKey;
+                for (int i = tid; i < (T >> 2); i += kSNT)  // T % 4 == 0: 16-byte stores
+                    reinterpret_cast<uint4*>(keys)[i] = make_uint4(kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey);
                 if (tid == 0) {
                     misc[5] = 0;
                     misc[6] = 0;
@@ -589,11 +590,12 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 // ids to scratch; the tile becomes the sums ("unset" until the first value)
                 {
                     uint32_t occ = 0;
-                    for (int i = tid; i < T; i += kSNT) {
-                        const uint32_t kv = keys[i];
-                        keys_g[i] = kv;
-                        keys[i] = kUnset;
-                        occ += kv != kEmptyKey ? 1u : 0u;
+                    for (int i = tid; i < (T >> 2); i += kSNT) {  // 16 bytes at a time
+                        const uint4 kv = reinterpret_cast<const uint4*>(keys)[i];
+                        reinterpret_cast<uint4*>(keys_g)[i] = kv;
+                        reinterpret_cast<uint4*>(keys)[i] = make_uint4(kUnset, kUnset, kUnset, kUnset);
+                        occ += (kv.x != kEmptyKey ? 1u : 0u) + (kv.y != kEmptyKey ? 1u : 0u) +
+                               (kv.z != kEmptyKey ? 1u : 0u) + (kv.w != kEmptyKey ? 1u : 0u);
                     }
                     occ = wave_sum(occ);
                     if (ln == 0) atomicAdd(reinterpret_cast<uint32_t*>(&misc[6]), occ);
@@ -713,13 +715,18 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             int shift;
             radix_select<kSNT>(NS, rs, [&](int i) -> uint64_t { return skey[i]; }, sh, prefix, shift, 32);
             const uint32_t lkey = (uint32_t)(prefix >> 32);
+            // the bound as a float: x >= Lf keeps every sum whose key reaches lkey (and -0
+            // against a +0 bound, a superset that the exact select below does not mind);
+            // kUnset is a NaN and compares false. A bound at or below the key of -inf keeps
+            // every sum.
+            const float Lf = lkey <= fkey(-INFINITY) ? -INFINITY : fkey_inv(lkey);
             for (int i0 = 0; i0 < T; i0 += kEmitU * kSNT) {  // uniform trip count (wave-level append)
                 uint32_t tm = 0;
 #pragma unroll
                 for (int u = 0; u < kEmitU; ++u) {
                     const int i = i0 + u * kSNT + tid;
                     const uint32_t sb = i < T ? keys[i] : kUnset;
-                    tm |= (sb != kUnset && fkey(__uint_as_float(sb)) >= lkey ? 1u : 0u) << u;
+                    tm |= (__uint_as_float(sb) >= Lf ? 1u : 0u) << u;
                 }
                 uint32_t pos = wave_append_n<4>((uint32_t)__popc(tm), reinterpret_cast<uint32_t*>(&misc[15]));
 #pragma unroll
@@ -1336,6 +1343,7 @@ hipError_t launch_coceos_search(const IndexDims& dm, const uint32_t* signs, cons
     const int DR = dm.D * dm.R;
     const int mp = std::max(1, qb.mp);
     if (mp > kSNT * kBufEntries) return hipErrorInvalidValue;
+    if (cfg.tile_buckets < 64 || search_tile_slots_nbk(cfg.tile_buckets) % 4 != 0) return hipErrorInvalidValue;  // 16-byte tile passes
     hipError_t err = hipSuccess;
     if (cfg.ev[0] && (err = hipEventRecord(cfg.ev[0], st)) != hipSuccess) return err;
     err = launch_project_rows(dm, qb.Q, 0, 1, qb.nq, signs, scr.qproj, DR, false, st, cfg.proj_waves, qb.nq_dev);
