@@ -538,6 +538,22 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             }
         };
 
+        // Tile sweeps, kEmitU = 8 slots per thread per step of 4096: the quads tid and
+        // tid + kSNT of the step, read 16 bytes at a time (T % 4 == 0); item u of a thread
+        // is slot tile_item(i0, u), slots past T read as `fill`.
+        static_assert(kEmitU == 8, "a sweep step reads two quads per thread");
+        auto tile_item = [&](int i0, int u) -> uint32_t {
+            return (uint32_t)(i0 + 4 * (tid + (u >> 2) * kSNT) + (u & 3));
+        };
+        auto tile_load8 = [&](const uint32_t* arr, int i0, uint32_t fill, uint32_t (&v)[kEmitU]) __attribute__((always_inline)) {
+            const int q0 = (i0 >> 2) + tid, q1 = q0 + kSNT, nq4 = T >> 2;
+            const uint4 f4 = make_uint4(fill, fill, fill, fill);
+            const uint4 a = q0 < nq4 ? reinterpret_cast<const uint4*>(arr)[q0] : f4;
+            const uint4 b = q1 < nq4 ? reinterpret_cast<const uint4*>(arr)[q1] : f4;
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+        };
+
         int ncand = 0;
         for (; sq > 0;) {
             if (tid == 0) { misc[2] = 0; misc[4] = 0; }
@@ -573,12 +589,9 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 if (hash_mode) {  // every id of this pass is a candidate (src/CEOs.cpp:1005-1026)
                     for (int i0 = 0; i0 < T; i0 += kEmitU * kSNT) {  // uniform trip count (wave-level append)
                         uint32_t kv[kEmitU], cnt = 0;
+                        tile_load8(keys, i0, kEmptyKey, kv);
 #pragma unroll
-                        for (int u = 0; u < kEmitU; ++u) {
-                            const int i = i0 + u * kSNT + tid;
-                            kv[u] = i < T ? keys[i] : kEmptyKey;
-                            cnt += kv[u] != kEmptyKey ? 1u : 0u;
-                        }
+                        for (int u = 0; u < kEmitU; ++u) cnt += kv[u] != kEmptyKey ? 1u : 0u;
                         uint32_t pos = wave_append_n<4>(cnt, reinterpret_cast<uint32_t*>(&misc[2]));
 #pragma unroll
                         for (int u = 0; u < kEmitU; ++u)
@@ -628,21 +641,16 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 }
                 // candidates of this pass: (sum, id) keys to scratch
                 for (int i0 = 0; i0 < T; i0 += kEmitU * kSNT) {  // uniform trip count (wave-level append)
-                    uint32_t idv[kEmitU];
+                    uint32_t idv[kEmitU], sv[kEmitU];
                     uint32_t cnt = 0;
-#pragma unroll
-                    for (int u = 0; u < kEmitU; ++u) {
-                        const int i = i0 + u * kSNT + tid;
-                        idv[u] = i < T ? keys_g[i] : kEmptyKey;
-                    }
+                    tile_load8(keys_g, i0, kEmptyKey, idv);
+                    tile_load8(keys, i0, kUnset, sv);
 #pragma unroll
                     for (int u = 0; u < kEmitU; ++u) cnt += idv[u] != kEmptyKey ? 1u : 0u;
                     uint32_t pos = wave_append_n<4>(cnt, reinterpret_cast<uint32_t*>(&misc[2]));
 #pragma unroll
-                    for (int u = 0; u < kEmitU; ++u) {
-                        const int i = i0 + u * kSNT + tid;
-                        if (idv[u] != kEmptyKey) cand[pos++] = ckey(__uint_as_float(keys[i]), idv[u]);
-                    }
+                    for (int u = 0; u < kEmitU; ++u)
+                        if (idv[u] != kEmptyKey) cand[pos++] = ckey(__uint_as_float(sv[u]), idv[u]);
                 }
                 __syncthreads();
             }
@@ -721,19 +729,15 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             // every sum.
             const float Lf = lkey <= fkey(-INFINITY) ? -INFINITY : fkey_inv(lkey);
             for (int i0 = 0; i0 < T; i0 += kEmitU * kSNT) {  // uniform trip count (wave-level append)
-                uint32_t tm = 0;
+                uint32_t tm = 0, sb[kEmitU];
+                tile_load8(keys, i0, kUnset, sb);
 #pragma unroll
-                for (int u = 0; u < kEmitU; ++u) {
-                    const int i = i0 + u * kSNT + tid;
-                    const uint32_t sb = i < T ? keys[i] : kUnset;
-                    tm |= (__uint_as_float(sb) >= Lf ? 1u : 0u) << u;
-                }
+                for (int u = 0; u < kEmitU; ++u) tm |= (__uint_as_float(sb[u]) >= Lf ? 1u : 0u) << u;
                 uint32_t pos = wave_append_n<4>((uint32_t)__popc(tm), reinterpret_cast<uint32_t*>(&misc[15]));
 #pragma unroll
-                for (int u = 0; u < kEmitU; ++u) {
-                    const int i = i0 + u * kSNT + tid;
-                    if ((tm >> u) & 1u) cand[pos++] = ((uint64_t)fkey(__uint_as_float(keys[i])) << 32) | (uint32_t)i;
-                }
+                for (int u = 0; u < kEmitU; ++u)
+                    if ((tm >> u) & 1u)
+                        cand[pos++] = ((uint64_t)fkey(__uint_as_float(sb[u])) << 32) | tile_item(i0, u);
             }
             __syncthreads();
             const int nk = misc[15];
@@ -793,9 +797,12 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 const float t = (__uint_as_float(sb) - vlo) * scale;
                 return t >= (float)(kSelBins - 1) ? kSelBins - 1 : (int)t;
             };
-            for (int i = tid; i < T; i += kSNT) {
-                const uint32_t sb = keys[i];
-                if (sb != kUnset) atomicAdd(&hist[vbin(sb)], 1u);
+            for (int q4 = tid; q4 < (T >> 2); q4 += kSNT) {  // 16 bytes at a time
+                const uint4 sb = reinterpret_cast<const uint4*>(keys)[q4];
+                if (sb.x != kUnset) atomicAdd(&hist[vbin(sb.x)], 1u);
+                if (sb.y != kUnset) atomicAdd(&hist[vbin(sb.y)], 1u);
+                if (sb.z != kUnset) atomicAdd(&hist[vbin(sb.z)], 1u);
+                if (sb.w != kUnset) atomicAdd(&hist[vbin(sb.w)], 1u);
             }
             __syncthreads();
             {  // boundary bin: thread t scans bins [kSelBins - BPT*(t+1), kSelBins - BPT*t) downwards
@@ -826,11 +833,11 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             const int bbin = misc[9], above = misc[10];
             // emission: slots above the boundary bin to cidl, boundary-bin slots to bslots
             for (int i0 = 0; i0 < T; i0 += kEmitU * kSNT) {  // uniform trip count (wave-level append)
-                uint32_t am = 0, bm = 0, ca = 0, cb = 0;
+                uint32_t am = 0, bm = 0, ca = 0, cb = 0, sbv[kEmitU];
+                tile_load8(keys, i0, kUnset, sbv);
 #pragma unroll
                 for (int u = 0; u < kEmitU; ++u) {
-                    const int i = i0 + u * kSNT + tid;
-                    const uint32_t sb = i < T ? keys[i] : kUnset;
+                    const uint32_t sb = sbv[u];
                     const int bn = sb != kUnset ? vbin(sb) : -1;
                     am |= (bn > bbin ? 1u : 0u) << u;
                     bm |= (bn == bbin ? 1u : 0u) << u;
@@ -841,7 +848,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 uint32_t pb = wave_append_n<4>(cb, reinterpret_cast<uint32_t*>(&misc[12]));
 #pragma unroll
                 for (int u = 0; u < kEmitU; ++u) {
-                    const uint32_t i = (uint32_t)(i0 + u * kSNT + tid);
+                    const uint32_t i = tile_item(i0, u);
                     if ((am >> u) & 1u) cidl[pa++] = i;
                     if ((bm >> u) & 1u) {
                         if (pb < (uint32_t)kBoundMax) bslots[pb] = i;
