@@ -7,16 +7,17 @@
 // parameters and arrival offsets, prepared outside the server lock and queued in one
 // step, and the arrival clock starts when it is queued. The worker admits due requests
 // from each submission's cursor, earliest arrival first. Whenever one of the
-// server's `inflight` batch slots is free, the worker takes every request that has
-// arrived (up to max_batch) into that slot: pinned host staging -> device buffers of a
-// captured search plan -> plan replay -> results back, all on the slot's own stream,
-// ending in an event. Requests that arrive while batches run are admitted into the
+// server's `inflight` batch slots is free, the worker takes the requests that have
+// arrived into that slot (up to max_batch; a backlog is shared among the free slots):
+// pinned host staging -> device buffers of a captured search plan -> plan replay ->
+// results back, all on the slot's own stream, ending in an event. Requests that arrive while batches run are admitted into the
 // next free slot, so batches overlap on the device (the next persistent grid fills
 // the CUs as the previous one drains) and with the host copies. The number of slots
 // is bounded by the HBM budget: each slot holds one plan's buffers and scratch.
 // streamCEOs updates and deletes are applied between batches: the worker waits for
 // the batches in flight, applies them in submission order, and the next batch sees
 // the new index.
+#include <algorithm>
 #include <condition_variable>
 #include <deque>
 #include <queue>
@@ -274,12 +275,42 @@ struct ceos_server {
     };
     std::vector<Taken> taken_;  // the batch being launched (worker thread only)
 
+    // Requests that have arrived by `now` and are not yet admitted, counted up to `limit`
+    // (mu held).
+    int64_t due_count(Clock::time_point now, int64_t limit) const {
+        int64_t n = 0;
+        for (const auto& f : feeds) {
+            if (!f->pending()) continue;
+            const auto first = f->order.begin() + f->next;
+            const auto last = std::partition_point(first, f->order.end(), [&](int32_t i) { return f->arrival(i) <= now; });
+            n += last - first;
+            if (n >= limit) break;
+        }
+        return n;
+    }
+
+    // Queries for the next batch. A backlog is shared among the free slots (at least
+    // kSplitMin each): rows are copied to the staging before a batch is launched, so one
+    // slot taking the whole backlog after all slots finished together would leave the
+    // device idle for that whole copy, and the next backlog would be larger still.
+    static constexpr int kSplitMin = 256;
+    int batch_cap(Clock::time_point now) const {
+        int free_slots = 0;
+        for (const auto& sl : slots) free_slots += !sl->busy;
+        if (free_slots <= 1) return cfg.max_batch;
+        const int64_t due = due_count(now, (int64_t)free_slots * cfg.max_batch);
+        if (due <= kSplitMin) return cfg.max_batch;
+        const int64_t share = (due + free_slots - 1) / free_slots;
+        return (int)std::min<int64_t>(cfg.max_batch, std::max<int64_t>(kSplitMin, share));
+    }
+
     void launch(Slot& sl, std::unique_lock<std::mutex>& lk) {
         const auto now = Clock::now();
         const int d = core->d;
         const float mp = (float)std::min(core->mp, core->m);
+        const int cap = batch_cap(now);
         taken_.clear();
-        while ((int)taken_.size() < cfg.max_batch) {
+        while ((int)taken_.size() < cap) {
             Feed* f = earliest();
             if (!f || f->next_arrival() > now) break;
             const int i = f->order[f->next];
