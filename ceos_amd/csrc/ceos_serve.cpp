@@ -10,9 +10,10 @@
 // server's `inflight` batch slots is free, the worker takes the requests that have
 // arrived into that slot (up to max_batch; a backlog is shared among the free slots):
 // pinned host staging -> device buffers of a captured search plan -> plan replay ->
-// results back, all on the slot's own stream, ending in an event. Requests that arrive while batches run are admitted into the
-// next free slot, so batches overlap on the device (the next persistent grid fills
-// the CUs as the previous one drains) and with the host copies. The number of slots
+// results back, all on the slot's own stream, ending in an event. Requests that arrive
+// while batches run are admitted into the next free slot, so batches overlap on the
+// device (the next persistent grid fills the CUs as the previous one drains) and with
+// the host copies. The number of slots
 // is bounded by the HBM budget: each slot holds one plan's buffers and scratch.
 // streamCEOs updates and deletes are applied between batches: the worker waits for
 // the batches in flight, applies them in submission order, and the next batch sees
@@ -20,6 +21,7 @@
 #include <algorithm>
 #include <condition_variable>
 #include <deque>
+#include <memory>
 #include <queue>
 #include <thread>
 #include <unordered_map>
@@ -52,6 +54,36 @@ struct Feed {
     }
     bool pending() const { return next < n; }
     Clock::time_point next_arrival() const { return arrival(order[next]); }
+};
+
+// Append-only record of fixed-size chunks that are kept across clear(): appending never
+// moves what is recorded (a vector doubling under the server lock copied megabytes into
+// freshly faulted pages and stalled the worker for milliseconds at 2^19, 2^20, ...
+// requests), and a cleared record refills its chunks without allocating.
+template <typename T>
+struct Record {
+    static constexpr size_t kChunk = size_t(1) << 15;
+    std::vector<std::unique_ptr<T[]>> chunks;
+    size_t n = 0;
+    void push_back(const T& v) {
+        if (n == chunks.size() * kChunk) chunks.emplace_back(new T[kChunk]);
+        chunks[n / kChunk][n % kChunk] = v;
+        ++n;
+    }
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+    void clear() { n = 0; }
+    const T& operator[](size_t i) const { return chunks[i / kChunk][i % kChunk]; }
+    const T& back() const { return (*this)[n - 1]; }
+    // The first m entries into out.
+    void copy_to(T* out, size_t m) const {
+        for (size_t i = 0; i < m; i += kChunk) std::copy_n(chunks[i / kChunk].get(), std::min(kChunk, m - i), out + i);
+    }
+    std::vector<T> to_vector() const {
+        std::vector<T> v(n);
+        copy_to(v.data(), n);
+        return v;
+    }
 };
 
 struct Update {
@@ -166,10 +198,10 @@ struct ceos_server {
 
     // statistics
     int64_t submitted = 0, completed = 0, batches = 0, batched_queries = 0;
-    std::vector<double> lat_ms;
-    std::vector<double> wait_ms;     // per request: arrival -> its batch launched
-    std::vector<double> service_ms;  // per batch: launch -> results in host memory
-    std::vector<ceos_server_batch> trace;  // per batch (ceos_server_get_batches), at most kMaxTrace
+    Record<double> lat_ms;
+    Record<double> wait_ms;     // per request: arrival -> its batch launched
+    Record<double> service_ms;  // per batch: launch -> results in host memory
+    Record<ceos_server_batch> trace;  // per batch (ceos_server_get_batches), at most kMaxTrace
     static constexpr size_t kMaxTrace = 1 << 20;
     int64_t n_upd[2] = {0, 0};                           // inserts, deletes applied
     double upd_ms_sum[2] = {0.0, 0.0}, upd_ms_max[2] = {0.0, 0.0};
@@ -719,23 +751,31 @@ int ceos_server_drain(ceos_server* s) {
 int ceos_server_get_stats(ceos_server* s, ceos_server_stats* out) {
     return guarded([&] {
         if (!s || !out) throw ArgError("null argument");
-        std::lock_guard<std::mutex> g(s->mu);
         ceos_server_stats st{};
-        st.submitted = s->submitted;
-        st.completed = s->completed;
-        st.batches = s->batches;
-        st.mean_batch = s->batches ? (double)s->batched_queries / (double)s->batches : 0.0;
-        std::vector<double> v = s->lat_ms;
-        if (!v.empty()) {
-            std::sort(v.begin(), v.end());
-            auto pct = [&](double p) { return v[std::min(v.size() - 1, (size_t)(p * (double)(v.size() - 1) + 0.5))]; };
-            st.p50_ms = pct(0.50);
-            st.p99_ms = pct(0.99);
-            st.max_ms = v.back();
-            st.elapsed_s = std::chrono::duration<double>(s->last_done - s->first_arrival).count();
-            st.qps = st.elapsed_s > 0 ? (double)s->completed / st.elapsed_s : 0.0;
+        std::vector<double> v, w, sv;
+        {  // copies under the lock; the sorting after it, so the worker is not held up
+            std::lock_guard<std::mutex> g(s->mu);
+            st.submitted = s->submitted;
+            st.completed = s->completed;
+            st.batches = s->batches;
+            st.mean_batch = s->batches ? (double)s->batched_queries / (double)s->batches : 0.0;
+            v = s->lat_ms.to_vector();
+            w = s->wait_ms.to_vector();
+            sv = s->service_ms.to_vector();
+            if (!v.empty()) {
+                st.elapsed_s = std::chrono::duration<double>(s->last_done - s->first_arrival).count();
+                st.qps = st.elapsed_s > 0 ? (double)s->completed / st.elapsed_s : 0.0;
+            }
+            st.inflight_slots = (int)s->slots.size();
+            st.device_bytes = s->device_bytes;
+            st.inserts = s->n_upd[0];
+            st.deletes = s->n_upd[1];
+            st.insert_mean_ms = s->n_upd[0] ? s->upd_ms_sum[0] / (double)s->n_upd[0] : 0.0;
+            st.insert_max_ms = s->upd_ms_max[0];
+            st.delete_mean_ms = s->n_upd[1] ? s->upd_ms_sum[1] / (double)s->n_upd[1] : 0.0;
+            st.delete_max_ms = s->upd_ms_max[1];
         }
-        auto pcts = [](std::vector<double> x, double& p50, double& p99, double& mx) {
+        auto pcts = [](std::vector<double>& x, double& p50, double& p99, double& mx) {
             if (x.empty()) return;
             std::sort(x.begin(), x.end());
             auto at = [&](double p) { return x[std::min(x.size() - 1, (size_t)(p * (double)(x.size() - 1) + 0.5))]; };
@@ -743,16 +783,9 @@ int ceos_server_get_stats(ceos_server* s, ceos_server_stats* out) {
             p99 = at(0.99);
             mx = x.back();
         };
-        pcts(s->wait_ms, st.wait_p50_ms, st.wait_p99_ms, st.wait_max_ms);
-        pcts(s->service_ms, st.service_p50_ms, st.service_p99_ms, st.service_max_ms);
-        st.inflight_slots = (int)s->slots.size();
-        st.device_bytes = s->device_bytes;
-        st.inserts = s->n_upd[0];
-        st.deletes = s->n_upd[1];
-        st.insert_mean_ms = s->n_upd[0] ? s->upd_ms_sum[0] / (double)s->n_upd[0] : 0.0;
-        st.insert_max_ms = s->upd_ms_max[0];
-        st.delete_mean_ms = s->n_upd[1] ? s->upd_ms_sum[1] / (double)s->n_upd[1] : 0.0;
-        st.delete_max_ms = s->upd_ms_max[1];
+        pcts(v, st.p50_ms, st.p99_ms, st.max_ms);
+        pcts(w, st.wait_p50_ms, st.wait_p99_ms, st.wait_max_ms);
+        pcts(sv, st.service_p50_ms, st.service_p99_ms, st.service_max_ms);
         *out = st;
     });
 }
@@ -763,7 +796,7 @@ int ceos_server_get_batches(ceos_server* s, ceos_server_batch* out, int64_t max,
         std::lock_guard<std::mutex> g(s->mu);
         *count = (int64_t)s->trace.size();
         const int64_t n = std::min<int64_t>(std::max<int64_t>(max, 0), (int64_t)s->trace.size());
-        if (n > 0) std::memcpy(out, s->trace.data(), (size_t)n * sizeof(ceos_server_batch));
+        if (n > 0) s->trace.copy_to(out, (size_t)n);
     });
 }
 
