@@ -10,7 +10,8 @@
 //    a. score accumulation in two phases over an LDS tile of 4 * tile_buckets hashed slots
 //       plus a small overflow region.
 //       Phase 1 gives every distinct id of the probed lists a slot (two-choice
-//       buckets of four 4-byte ids, claims by atomicMin between barriers, a small
+//       buckets of four 4-byte ids, or one bucket per id when the pass is planned
+//       at most 55% full; claims by atomicMin between barriers; a small
 //       linear-probing stash for the rare leftovers) and records each bucket
 //       position's slot. Phase 2 turns the tile into FP32 sums and adds the values
 //       probe by probe, a barrier between probes, so each point's sum is formed in
@@ -47,6 +48,7 @@ constexpr int kSWaves = kSNT / kWave;
 constexpr int kPNT = 256;              // probe-select block
 constexpr uint32_t kEmptyKey = 0xffffffffu;
 constexpr int kBufEntries = 8;         // bucket entries per thread per chunk
+constexpr float kOneChoiceLoad = 0.55f;  // planned load of a single-bucket pass (phase 1)
 constexpr uint32_t kUnset = 0xffffffffu;       // sum bits of a slot before its first (ordered) add
 typedef const float __attribute__((address_space(1)))* GlobalF;
 typedef float F4v __attribute__((ext_vector_type(4)));
@@ -391,8 +393,8 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         // list. Copies of one id in a chunk decide alike (same table, same rule), so they
         // claim the same slot and win or lose together. All reads of a chunk are issued
         // before any of them is used.
-        auto phase1_chunk = [&](const Entry (&buf)[CH][EPT], int j0, uint32_t pass) {
-            uint32_t id[B], slot[B], b1[B], b2[B], hh[B];
+        auto phase1_chunk = [&](const Entry (&buf)[CH][EPT], int j0, uint32_t pass, bool one) {
+            uint32_t id[B], slot[B], b1[B], hh[B];
             float v[B];
             // the chunk's probe lengths (0 past the last probe), read once
             int limc[CH];
@@ -410,42 +412,70 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 const bool valid = pos_ok && (P == 1 || pass_of(hh[b], P) == pass);
                 slot[b] = valid ? kToPlace : kEmptyKey;  // kEmptyKey: not in this pass
                 b1[b] = bucket1(hh[b], nbk);
-                b2[b] = bucket2(hh[b], b1[b], nbk);
-            }
-            uint4 k1[B], k2[B];
-#pragma unroll
-            for (int b = 0; b < B; ++b) {
-                k1[b] = reinterpret_cast<const uint4*>(keys)[b1[b]];
-                k2[b] = reinterpret_cast<const uint4*>(keys)[b2[b]];
             }
             uint32_t claim = 0;
+            if (one) {
+                // One bucket per id (a pass planned well below the tile's capacity): an id
+                // present in its bucket takes its slot, a new one claims an empty slot of
+                // it; a full bucket sends the entry to the stash list.
+                uint4 k1[B];
 #pragma unroll
-            for (int b = 0; b < B; ++b) {
-                const uint32_t x = id[b];
-                const uint4 p = k1[b], q2 = k2[b];
-                // one-hot match masks (ids within a bucket are distinct)
-                const uint32_t m1 = (p.x == x ? 1u : 0u) | (p.y == x ? 2u : 0u) | (p.z == x ? 4u : 0u) |
-                                    (p.w == x ? 8u : 0u);
-                const uint32_t m2 = (q2.x == x ? 1u : 0u) | (q2.y == x ? 2u : 0u) | (q2.z == x ? 4u : 0u) |
-                                    (q2.w == x ? 8u : 0u);
-                // empty masks: ids are below 2^31, the empty key has the top bit set
-                const uint32_t e1 = (p.x >> 31) | ((p.y >> 30) & 2u) | ((p.z >> 29) & 4u) | ((p.w >> 28) & 8u);
-                const uint32_t e2 = (q2.x >> 31) | ((q2.y >> 30) & 2u) | ((q2.z >> 29) & 4u) | ((q2.w >> 28) & 8u);
-                const bool in1 = m1 != 0u || (m2 == 0u && e1 != 0u);  // target bucket: b1 or b2
-                const uint32_t hm = in1 ? m1 : m2;
-                // a new id takes the first empty slot at or after a per-id start position
-                // (cyclically), so that different ids rarely claim the same slot
-                const uint32_t r = (hh[b] >> 4) & 3u;
-                const uint32_t em = in1 ? e1 : e2;
-                const uint32_t rot = ((em | (em << 4)) >> r) & 15u;
-                const uint32_t within = hm != 0u ? (uint32_t)__builtin_ctz(hm | 16u)
-                                                 : (((uint32_t)__builtin_ctz(rot | 16u) + r) & 3u);
-                const uint32_t pos = 4u * (in1 ? b1[b] : b2[b]) + within;
-                const bool hit = (m1 | m2) != 0u;
-                const bool room = (e1 | e2) != 0u;
-                const bool place = slot[b] == kToPlace;
-                slot[b] = place ? (hit || room ? pos : kStashMark) : slot[b];
-                claim |= (place && !hit && room ? 1u : 0u) << b;
+                for (int b = 0; b < B; ++b) k1[b] = reinterpret_cast<const uint4*>(keys)[b1[b]];
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const uint32_t x = id[b];
+                    const uint4 p = k1[b];
+                    const uint32_t m1 = (p.x == x ? 1u : 0u) | (p.y == x ? 2u : 0u) | (p.z == x ? 4u : 0u) |
+                                        (p.w == x ? 8u : 0u);
+                    const uint32_t e1 = (p.x >> 31) | ((p.y >> 30) & 2u) | ((p.z >> 29) & 4u) | ((p.w >> 28) & 8u);
+                    const uint32_t r = (hh[b] >> 4) & 3u;
+                    const uint32_t rot = ((e1 | (e1 << 4)) >> r) & 15u;
+                    const uint32_t within = m1 != 0u ? (uint32_t)__builtin_ctz(m1 | 16u)
+                                                     : (((uint32_t)__builtin_ctz(rot | 16u) + r) & 3u);
+                    const uint32_t pos = 4u * b1[b] + within;
+                    const bool hit = m1 != 0u;
+                    const bool room = e1 != 0u;
+                    const bool place = slot[b] == kToPlace;
+                    slot[b] = place ? (hit || room ? pos : kStashMark) : slot[b];
+                    claim |= (place && !hit && room ? 1u : 0u) << b;
+                }
+            } else {
+                uint32_t b2[B];
+                uint4 k1[B], k2[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    b2[b] = bucket2(hh[b], b1[b], nbk);
+                    k1[b] = reinterpret_cast<const uint4*>(keys)[b1[b]];
+                    k2[b] = reinterpret_cast<const uint4*>(keys)[b2[b]];
+                }
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const uint32_t x = id[b];
+                    const uint4 p = k1[b], q2 = k2[b];
+                    // one-hot match masks (ids within a bucket are distinct)
+                    const uint32_t m1 = (p.x == x ? 1u : 0u) | (p.y == x ? 2u : 0u) | (p.z == x ? 4u : 0u) |
+                                        (p.w == x ? 8u : 0u);
+                    const uint32_t m2 = (q2.x == x ? 1u : 0u) | (q2.y == x ? 2u : 0u) | (q2.z == x ? 4u : 0u) |
+                                        (q2.w == x ? 8u : 0u);
+                    // empty masks: ids are below 2^31, the empty key has the top bit set
+                    const uint32_t e1 = (p.x >> 31) | ((p.y >> 30) & 2u) | ((p.z >> 29) & 4u) | ((p.w >> 28) & 8u);
+                    const uint32_t e2 = (q2.x >> 31) | ((q2.y >> 30) & 2u) | ((q2.z >> 29) & 4u) | ((q2.w >> 28) & 8u);
+                    const bool in1 = m1 != 0u || (m2 == 0u && e1 != 0u);  // target bucket: b1 or b2
+                    const uint32_t hm = in1 ? m1 : m2;
+                    // a new id takes the first empty slot at or after a per-id start position
+                    // (cyclically), so that different ids rarely claim the same slot
+                    const uint32_t r = (hh[b] >> 4) & 3u;
+                    const uint32_t em = in1 ? e1 : e2;
+                    const uint32_t rot = ((em | (em << 4)) >> r) & 15u;
+                    const uint32_t within = hm != 0u ? (uint32_t)__builtin_ctz(hm | 16u)
+                                                     : (((uint32_t)__builtin_ctz(rot | 16u) + r) & 3u);
+                    const uint32_t pos = 4u * (in1 ? b1[b] : b2[b]) + within;
+                    const bool hit = (m1 | m2) != 0u;
+                    const bool room = (e1 | e2) != 0u;
+                    const bool place = slot[b] == kToPlace;
+                    slot[b] = place ? (hit || room ? pos : kStashMark) : slot[b];
+                    claim |= (place && !hit && room ? 1u : 0u) << b;
+                }
             }
             __syncthreads();  // every read of this chunk precedes every claim of this chunk
 #pragma unroll
@@ -555,7 +585,12 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         };
 
         int ncand = 0;
+        // expected distinct ids, as the partitions were planned
+        const float exp_ids = (float)E * fminf(1.0f, frac * (first ? 1.25f : 1.0f));
         for (; sq > 0;) {
+            // a single pass expected to fill at most kOneChoiceLoad of the main slots uses
+            // one bucket per id: ids over a full bucket are few there, and the stash takes them
+            const bool one = P == 1 && exp_ids <= kOneChoiceLoad * (float)Tmain;
             if (tid == 0) { misc[2] = 0; misc[4] = 0; }
             bool overflow = false;
             for (uint32_t pass = 0; pass < P && !overflow; ++pass) {
@@ -574,10 +609,10 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     load_chunk(bufA, 0);
                     for (int j0 = 0; j0 < sq; j0 += 2 * CH) {
                         load_chunk(bufB, j0 + CH);
-                        phase1_chunk(bufA, j0, pass);
+                        phase1_chunk(bufA, j0, pass, one);
                         if (j0 + CH >= sq) break;
                         load_chunk(bufA, j0 + 2 * CH);
-                        phase1_chunk(bufB, j0 + CH, pass);
+                        phase1_chunk(bufB, j0 + CH, pass, one);
                     }
                 }
                 __syncthreads();

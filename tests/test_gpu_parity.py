@@ -307,3 +307,21 @@ def test_search_tile_sizes(CEOs, oracle, wg, tl):
     gi, gp = g.search_coCEOs_Est(Q, 10)
     oi, op = o.search_coCEOs_Est(Q, 10)
     _compare_search(gi, gp, oi, op, tol_frac=0.0)
+
+
+@pytest.mark.parametrize("s", [3, 5, 6, 7, 9])
+def test_search_single_bucket_tile_load(CEOs, oracle, s):
+    """A tile of 2^14 slots and about 1,500 distinct ids per probe: the passes planned at
+    most 55% full place each id in one bucket, so the fuller ones send the ids of full
+    buckets to the stash; fuller passes take both buckets or several partitions. Every
+    load gives the oracle's ids."""
+    X = _rand(60_000, 96, 61)
+    Q = _rand(150, 96, 62)
+    g, o = _build_pair(CEOs, oracle, X, 256, 2, 1500)
+    g.set_search_config(1, 14)
+    for obj in (g, o):
+        obj.n_probed_vectors = s
+        obj.n_cand = 500
+    gi, gp = g.search_coCEOs_Est(Q, 10)
+    oi, op = o.search_coCEOs_Est(Q, 10)
+    _compare_search(gi, gp, oi, op, tol_frac=0.0)
