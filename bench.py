@@ -468,7 +468,8 @@ def main():
         trace = srv.batches()
         srv.close()
         slow = [{"launch_s": round(float(b[0]), 4), "enqueue_ms": round(float(b[1]), 3), "done_ms": round(float(b[2]), 3),
-                 "oldest_wait_ms": round(float(b[3]), 3), "queries": int(b[4])}
+                 "oldest_wait_ms": round(float(b[3]), 3), "queries": int(b[4]),
+                 "enqueue_steps_ms": [round(float(x), 3) for x in b[6:11]]}
                 for b in trace[np.argsort(-trace[:, 2])[:3]]] if len(trace) else []
         served = {"offered_qps": rate, "achieved_qps": round(st["qps"], 1), "requests": nreq,
                   "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),

@@ -11,7 +11,7 @@
 //       plus a small overflow region.
 //       Phase 1 gives every distinct id of the probed lists a slot (two-choice
 //       buckets of four 4-byte ids, or one bucket per id when the pass is planned
-//       at most 55% full; claims by atomicMin between barriers; a small
+//       at most 70% full; claims by atomicMin between barriers; a small
 //       linear-probing stash for the rare leftovers) and records each bucket
 //       position's slot. Phase 2 turns the tile into FP32 sums and adds the values
 //       probe by probe, a barrier between probes, so each point's sum is formed in
@@ -48,7 +48,7 @@ constexpr int kSWaves = kSNT / kWave;
 constexpr int kPNT = 256;              // probe-select block
 constexpr uint32_t kEmptyKey = 0xffffffffu;
 constexpr int kBufEntries = 8;         // bucket entries per thread per chunk
-constexpr float kOneChoiceLoad = 0.55f;  // planned load of a single-bucket pass (phase 1)
+constexpr float kOneChoiceLoad = 0.70f;  // planned load of a single-bucket pass (phase 1)
 constexpr uint32_t kUnset = 0xffffffffu;       // sum bits of a slot before its first (ordered) add
 typedef const float __attribute__((address_space(1)))* GlobalF;
 typedef float F4v __attribute__((ext_vector_type(4)));
