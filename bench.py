@@ -378,13 +378,13 @@ def main():
     _, tile_log2, tile_lds = index.get_search_config()
     index.set_search_config(0, 0)
 
-    # ---- the practical streaming-read rate of this GPU: a plain grid-stride 16-byte read
-    # over 4 GiB, timed with HIP events on the same stream, at 2048, 4096 and 8192
-    # workgroups of 256; the best of the three
+    # ---- the practical streaming-read rate of this GPU: a plain grid-stride read of 4 GiB
+    # (16-byte non-temporal loads, 8 in flight per thread), timed with HIP events on the
+    # same stream, at 2048 to 16384 workgroups of 256; the best of the four
     xr = torch.ones(1 << 30, dtype=torch.float32, device=dev)
-    orr = torch.empty(8192 * 256, dtype=torch.float32, device=dev)
+    orr = torch.empty(16384 * 256, dtype=torch.float32, device=dev)
     read_by_grid = {}
-    for g in (2048, 4096, 8192):
+    for g in (2048, 4096, 8192, 16384):
         r = ctypes.c_double()
         capi.check(L.ceos_read_bandwidth(xr.data_ptr(), xr.numel(), orr.data_ptr(), g * 256, 10, stream,
                                          ctypes.byref(r)))
