@@ -488,15 +488,22 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
 #pragma unroll
             for (int b = 0; b < B; ++b)
                 if (((claim >> b) & 1u) && got[b] != id[b]) slot[b] = kStashMark;  // lost the claim
+            // the chunk's stash entries: one append per wave (a returning LDS atomic per
+            // entry would serialise up to B round trips per wave)
+            uint32_t smask = 0;
+#pragma unroll
+            for (int b = 0; b < B; ++b) smask |= (slot[b] == kStashMark ? 1u : 0u) << b;
+            static_assert(B < 16, "stash entries per thread fit 4 bits");
+            uint32_t sp = wave_append_n<4>((uint32_t)__builtin_popcount(smask), reinterpret_cast<uint32_t*>(&misc[5]));
 #pragma unroll
             for (int b = 0; b < B; ++b) {
                 const int j = j0 + b / EPT, t = tid + (b % EPT) * kSNT;
-                if (slot[b] == kStashMark) {
-                    const uint32_t sp = atomicAdd(reinterpret_cast<uint32_t*>(&misc[5]), 1u);
+                if ((smask >> b) & 1u) {
                     if (sp < (uint32_t)kStashList)
                         stash_list[sp] = make_uint4(id[b], __float_as_uint(v[b]), (uint32_t)(j * mp + t), 0u);
                     else
                         misc[4] = 1;  // stash list full: the pass restarts with more partitions
+                    ++sp;
                 }
                 if (!hash_mode && ((inpos >> b) & 1u))
                     pairs[(int64_t)j * mp + t] = make_uint2(slot[b] == kStashMark ? kEmptyKey : slot[b],
