@@ -512,6 +512,26 @@ __global__ __launch_bounds__(kWave) void k_bucket_purge(int m, Entry* __restrict
     if (ln == 0) size[list] = out;
 }
 
+// Sets the dead bit of each listed id's ring slot (ids are live logical ids, checked by
+// the caller; repeats are harmless).
+__global__ __launch_bounds__(256) void k_mark_dead(const int32_t* __restrict__ ids, int64_t count, int64_t head,
+                                                   int64_t cap, uint32_t* __restrict__ dead) {
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += step) {
+        int64_t sl = head + ids[i];
+        if (sl >= cap) sl -= cap;
+        atomicOr(&dead[sl >> 5], 1u << (sl & 31));
+    }
+}
+
+hipError_t launch_mark_dead(const int32_t* ids, int64_t count, int64_t head, int64_t cap, uint32_t* dead,
+                            hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    const int grid = (int)std::min<int64_t>(1024, (count + 255) / 256);
+    hipLaunchKernelGGL(k_mark_dead, dim3(grid), dim3(256), 0, st, ids, count, head, cap, dead);
+    return hipGetLastError();
+}
+
 hipError_t launch_bucket_purge(int nlists, int m, Entry* bkt, int32_t* size, const uint32_t* dead,
                                RowRing ring, hipStream_t st) {
     hipLaunchKernelGGL(k_bucket_purge, dim3(nlists), dim3(kWave), 0, st, m, bkt, size, dead, ring);

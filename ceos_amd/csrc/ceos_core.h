@@ -125,6 +125,46 @@ struct HBuf {
     }
 };
 
+// Pinned host buffer mapped into the device's address space and coherent: a kernel
+// reads what the host wrote before its launch (through `dp`), with no copy-engine
+// transfer.
+template <typename T>
+struct MappedBuf {
+    T* p = nullptr;
+    T* dp = nullptr;
+    size_t n = 0;
+    MappedBuf() = default;
+    MappedBuf(const MappedBuf&) = delete;
+    MappedBuf& operator=(const MappedBuf&) = delete;
+    ~MappedBuf() { release(); }
+    void ensure(size_t count) {
+        if (count <= n) return;
+        release();
+        size_t cap = 1024;
+        while (cap < count) cap *= 2;
+        if (hipHostMalloc(reinterpret_cast<void**>(&p), cap * sizeof(T), hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            throw MemError("pinned host allocation of " + std::to_string(cap * sizeof(T)) + " bytes failed");
+        }
+        void* d = nullptr;
+        if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess || !d) {
+            (void)hipGetLastError();
+            (void)hipHostFree(p);
+            p = nullptr;
+            throw MemError("pinned host memory is not mapped for the device");
+        }
+        dp = static_cast<T*>(d);
+        n = cap;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = dp = nullptr;
+        n = 0;
+    }
+};
+
 // src/CEOs.h:76-79
 inline int fht_dim(int n_proj, int d) {
     if (n_proj < d) return 1 << (int)std::ceil(std::log2((double)d));
@@ -230,6 +270,7 @@ struct Core {
     int64_t cap = 0, head = 0;
     uint32_t base = 0, live = 0;
     DBuf<uint32_t> dead;  // stream random-delete bitmap over ring slots
+    MappedBuf<int32_t> del_ids;  // ids of a random delete, read by the marking kernel
     // rerank bounds: int8 row copies [cap][quant_stride(d)] holding each row's codes and
     // (s, r1, r2, 0), kept in step with X for coCEOs and stream indexes (see
     // k_quantize_rows); off: not kept, every candidate gets the exact rerank

@@ -86,6 +86,8 @@ hipError_t launch_stream_merge(int DR, int m, Entry* bkt, int32_t* size, const f
                                hipStream_t st);
 
 // Removes entries whose row is flagged in `dead` (bitmap over ring slots).
+hipError_t launch_mark_dead(const int32_t* ids, int64_t count, int64_t head, int64_t cap, uint32_t* dead,
+                            hipStream_t st);
 hipError_t launch_bucket_purge(int nlists, int m, Entry* bkt, int32_t* size, const uint32_t* dead,
                                RowRing ring, hipStream_t st);
 

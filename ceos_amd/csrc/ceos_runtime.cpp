@@ -743,16 +743,14 @@ int ceos_stream_delete_ids(ceos_stream* h, const int32_t* ids, int64_t count) {
         if (!c.built) throw StateError("stream index is not built");
         if (count < 0 || (!ids && count > 0)) throw ArgError("invalid id list");
         if (count == 0) return;
-        c.wait_reads();
-        std::vector<uint32_t> dh(c.dead.n);
-        HIP_CHECK(hipMemcpy(dh.data(), c.dead.p, dh.size() * 4, hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < count; ++i) {
+        for (int64_t i = 0; i < count; ++i)
             if (ids[i] < 0 || (uint32_t)ids[i] >= c.live) throw ArgError("id " + std::to_string(ids[i]) + " is not live");
-            int64_t sl = c.head + ids[i];
-            if (sl >= c.cap) sl -= c.cap;
-            dh[sl >> 5] |= 1u << (sl & 31);
-        }
-        HIP_CHECK(hipMemcpyAsync(c.dead.p, dh.data(), dh.size() * 4, hipMemcpyHostToDevice, c.st()));
+        c.wait_reads();
+        // the ids go to mapped pinned memory and a kernel sets their bits: the bitmap
+        // never crosses PCIe, and no copy-engine transfer is queued
+        c.del_ids.ensure((size_t)count);
+        std::memcpy(c.del_ids.p, ids, (size_t)count * sizeof(int32_t));
+        HIP_CHECK(ceos::launch_mark_dead(c.del_ids.dp, count, c.head, c.cap, c.dead.p, c.st()));
         HIP_CHECK(ceos::launch_bucket_purge(2 * c.DR(), c.m, c.bkt.p, c.bsize.p, c.dead.p, c.rows(), c.st()));
         HIP_CHECK(hipStreamSynchronize(c.st()));
     });
