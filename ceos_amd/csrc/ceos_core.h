@@ -271,6 +271,12 @@ struct Core {
     uint32_t base = 0, live = 0;
     DBuf<uint32_t> dead;  // stream random-delete bitmap over ring slots
     MappedBuf<int32_t> del_ids;  // ids of a random delete, read by the marking kernel
+    // stream update scratch (the new rows' projections, row-major then per list), kept
+    // between updates up to kUpdScratchKeep floats each: a 10,000-row update at
+    // n_proj*n_repeats 4096 needs 164 MB of each, and allocating and freeing them took
+    // 0.4 of the update's 0.58 ms projection phase
+    static constexpr size_t kUpdScratchKeep = size_t(64) << 20;
+    DBuf<float> upd_rows, upd_lists;
     // rerank bounds: int8 row copies [cap][quant_stride(d)] holding each row's codes and
     // (s, r1, r2, 0), kept in step with X for coCEOs and stream indexes (see
     // k_quantize_rows); off: not kept, every candidate gets the exact rerank
@@ -466,6 +472,7 @@ struct Core {
         dcenter.release();
         center.clear();
         bkt.release(); bsize.release(); X.release(); signs.release(); dead.release();
+        upd_rows.release(); upd_lists.release();
         Xq.release();
         scr.release(); estP.release();
         bw_rows.release(); bw_sample.release(); bw_thr.release(); bw_base.release(); bw_cap.release();

@@ -703,28 +703,38 @@ int ceos_stream_update(ceos_stream* h, const float* Xn, int64_t n_new, int d, in
             c.quantize_rows(slot0, n_new, n_cur + n_new, st);
         }
         // project the new rows: column-major [DR][n_new], through a row-major projection
-        // (coalesced stores) and an LDS-tiled transpose
-        DBuf<float> np((size_t)c.DR() * std::max<int64_t>(n_new, 1));
+        // (coalesced stores) and an LDS-tiled transpose; scratch kept on the index for
+        // updates up to kUpdScratchKeep floats, allocated for this update above that
+        const size_t proj_n = (size_t)c.DR() * std::max<int64_t>(n_new, 1);
+        const bool keep = proj_n <= Core::kUpdScratchKeep;
+        DBuf<float> np_own, tmp_own;
+        if (keep) {
+            c.upd_lists.ensure(proj_n);
+        } else {
+            np_own.alloc(proj_n);
+        }
+        float* np_p = keep ? c.upd_lists.p : np_own.p;
         if (n_new > 0 && n_new <= int64_t(65535) * 64) {
-            DBuf<float> tmp((size_t)c.DR() * n_new);
-            HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p + slot0 * c.d, 0, 1, seg1, c.signs.p, tmp.p, c.DR(),
+            if (keep) c.upd_rows.ensure(proj_n); else tmp_own.alloc(proj_n);
+            float* tmp_p = keep ? c.upd_rows.p : tmp_own.p;
+            HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p + slot0 * c.d, 0, 1, seg1, c.signs.p, tmp_p, c.DR(),
                                                 false, st));
             if (n_new > seg1)
                 HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p, 0, 1, n_new - seg1, c.signs.p,
-                                                    tmp.p + seg1 * c.DR(), c.DR(), false, st));
-            HIP_CHECK(ceos::launch_transpose(tmp.p, n_new, c.DR(), np.p, st, n_new));
-            HIP_CHECK(hipStreamSynchronize(st));  // tmp is freed on return
+                                                    tmp_p + seg1 * c.DR(), c.DR(), false, st));
+            HIP_CHECK(ceos::launch_transpose(tmp_p, n_new, c.DR(), np_p, st, n_new));
+            if (!keep) HIP_CHECK(hipStreamSynchronize(st));  // tmp_own is freed on return
         } else if (n_new > 0) {
-            HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p + slot0 * c.d, 0, 1, seg1, c.signs.p, np.p, n_new,
+            HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p + slot0 * c.d, 0, 1, seg1, c.signs.p, np_p, n_new,
                                                 true, st));
             if (n_new > seg1)
-                HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p, 0, 1, n_new - seg1, c.signs.p, np.p + seg1,
+                HIP_CHECK(ceos::launch_project_rows(c.dims(), c.X.p, 0, 1, n_new - seg1, c.signs.p, np_p + seg1,
                                                     n_new, true, st));
         }
         const uint32_t new_base = c.base + (uint32_t)n_del;
         const uint32_t new_live = (uint32_t)(n_cur + n_new - n_del);
         c.phase_mark(2, st);
-        HIP_CHECK(ceos::launch_stream_merge(c.DR(), c.m, c.bkt.p, c.bsize.p, np.p, n_new, first_abs, new_base,
+        HIP_CHECK(ceos::launch_stream_merge(c.DR(), c.m, c.bkt.p, c.bsize.p, np_p, n_new, first_abs, new_base,
                                             new_live, st));
         c.phase_mark(3, st);
         HIP_CHECK(hipStreamSynchronize(st));
