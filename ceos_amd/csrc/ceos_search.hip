@@ -477,17 +477,31 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     claim |= (place && !hit && room ? 1u : 0u) << b;
                 }
             }
-            __syncthreads();  // every read of this chunk precedes every claim of this chunk
+            if (one) {
+                // Claims by compare-and-swap, with no barrier: a claim that finds the same
+                // id has its slot; one that finds another id goes to the stash. Slots only
+                // fill within a pass, so an id never gets two bucket slots, and the stash
+                // pass finds in the bucket any id whose copy won a slot there.
+                uint32_t got[B];
 #pragma unroll
-            for (int b = 0; b < B; ++b)
-                if ((claim >> b) & 1u) atomicMin(&keys[slot[b]], id[b]);
-            __syncthreads();  // claims are complete before they are read back
-            uint32_t got[B];
+                for (int b = 0; b < B; ++b)
+                    got[b] = ((claim >> b) & 1u) ? atomicCAS(&keys[slot[b]], kEmptyKey, id[b]) : kEmptyKey;
 #pragma unroll
-            for (int b = 0; b < B; ++b) got[b] = keys[((claim >> b) & 1u) ? slot[b] : 0u];
+                for (int b = 0; b < B; ++b)
+                    if (got[b] != kEmptyKey && got[b] != id[b]) slot[b] = kStashMark;  // taken by another id
+            } else {
+                __syncthreads();  // every read of this chunk precedes every claim of this chunk
 #pragma unroll
-            for (int b = 0; b < B; ++b)
-                if (((claim >> b) & 1u) && got[b] != id[b]) slot[b] = kStashMark;  // lost the claim
+                for (int b = 0; b < B; ++b)
+                    if ((claim >> b) & 1u) atomicMin(&keys[slot[b]], id[b]);
+                __syncthreads();  // claims are complete before they are read back
+                uint32_t got[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) got[b] = keys[((claim >> b) & 1u) ? slot[b] : 0u];
+#pragma unroll
+                for (int b = 0; b < B; ++b)
+                    if (((claim >> b) & 1u) && got[b] != id[b]) slot[b] = kStashMark;  // lost the claim
+            }
             // the chunk's stash entries: one append per wave (a returning LDS atomic per
             // entry would serialise up to B round trips per wave)
             uint32_t smask = 0;
