@@ -11,7 +11,7 @@
 //       plus a small overflow region.
 //       Phase 1 gives every distinct id of the probed lists a slot (two-choice
 //       buckets of four 4-byte ids, or one bucket per id when the pass is planned
-//       at most 70% full; claims by atomicMin between barriers; a small
+//       at most 70% full; claims by compare-and-swap, with no barrier; a small
 //       linear-probing stash for the rare leftovers) and records each bucket
 //       position's slot. Phase 2 turns the tile into FP32 sums and adds the values
 //       probe by probe, a barrier between probes, so each point's sum is formed in
@@ -385,14 +385,12 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                 for (int e = 0; e < EPT; ++e) buf[jj][e] = pl[min(tid + e * kSNT, mp - 1)];
             }
         };
-        // One chunk of phase 1, two barriers. Every entry reads both of its buckets; an id
-        // already present takes its slot; otherwise it claims an empty slot of the first
-        // bucket with room (atomicMin, after a barrier so that every read of the chunk has seen
-        // the same table); after a second barrier each claim is read back, and the winners
-        // take their slot. Losers and ids whose buckets are both full go to the stash
-        // list. Copies of one id in a chunk decide alike (same table, same rule), so they
-        // claim the same slot and win or lose together. All reads of a chunk are issued
-        // before any of them is used.
+        // One chunk of phase 1, no barrier. Every entry reads its bucket (both buckets in
+        // a two-bucket pass); an id already present takes its slot; otherwise it claims an
+        // empty slot of the first bucket with room by compare-and-swap. A claim that finds
+        // the same id (a copy of it claimed first) has its slot; losers, and ids whose
+        // buckets are full, go to the stash list, whose pass looks for each id in its
+        // buckets first. All reads of a chunk are issued before any of them is used.
         auto phase1_chunk = [&](const Entry (&buf)[CH][EPT], int j0, uint32_t pass, bool one) {
             uint32_t id[B], slot[B], b1[B], hh[B];
             float v[B];
@@ -477,11 +475,12 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
                     claim |= (place && !hit && room ? 1u : 0u) << b;
                 }
             }
-            if (one) {
+            {
                 // Claims by compare-and-swap, with no barrier: a claim that finds the same
                 // id has its slot; one that finds another id goes to the stash. Slots only
-                // fill within a pass, so an id never gets two bucket slots, and the stash
-                // pass finds in the bucket any id whose copy won a slot there.
+                // fill within a pass, so an id never gets two bucket slots (with two
+                // buckets, an id goes to its second only while its first is full), and
+                // the stash pass finds in the buckets any id whose copy won a slot there.
                 uint32_t got[B];
 #pragma unroll
                 for (int b = 0; b < B; ++b)
@@ -489,18 +488,6 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
 #pragma unroll
                 for (int b = 0; b < B; ++b)
                     if (got[b] != kEmptyKey && got[b] != id[b]) slot[b] = kStashMark;  // taken by another id
-            } else {
-                __syncthreads();  // every read of this chunk precedes every claim of this chunk
-#pragma unroll
-                for (int b = 0; b < B; ++b)
-                    if ((claim >> b) & 1u) atomicMin(&keys[slot[b]], id[b]);
-                __syncthreads();  // claims are complete before they are read back
-                uint32_t got[B];
-#pragma unroll
-                for (int b = 0; b < B; ++b) got[b] = keys[((claim >> b) & 1u) ? slot[b] : 0u];
-#pragma unroll
-                for (int b = 0; b < B; ++b)
-                    if (((claim >> b) & 1u) && got[b] != id[b]) slot[b] = kStashMark;  // lost the claim
             }
             // the chunk's stash entries: one append per wave (a returning LDS atomic per
             // entry would serialise up to B round trips per wave)
