@@ -34,6 +34,30 @@ def _alone(g, q, k, s, c):
     return ids, ips
 
 
+def test_serve_large_batches(CEOs, index):
+    """3,000 requests at once into slots of 2,048: a backlog shared among the free slots
+    gives batches of 512 requests or more. Every sampled result equals the query
+    searched alone."""
+    X, g = index
+    n = 3000
+    Q = _rand(n, 64, 58)
+    rng = np.random.default_rng(59)
+    k = rng.choice(np.array([1, 10, 100], np.int32), n).astype(np.int32)
+    s = rng.integers(10, 81, n).astype(np.int32)
+    c = rng.integers(50, 3001, n).astype(np.int32)
+    srv = CEOs.Server(g, max_batch=2048, k_max=100, s_max=80, c_max=3000, inflight=2)
+    t0 = srv.submit(Q, k, s, c)
+    got = [srv.result(t0 + i) for i in range(n)]
+    trace = srv.batches()
+    srv.close()
+    assert trace[:, 4].max() >= 512
+    for i in rng.choice(n, 200, replace=False):
+        ids, ips, _ = got[i]
+        a, p = _alone(g, Q[i], int(k[i]), int(s[i]), int(c[i]))
+        assert np.array_equal(ids, a), i
+        assert np.array_equal(bits(ips), bits(p)), i
+
+
 def test_serve_open_loop_each_result_equals_alone(CEOs, index):
     X, g = index
     n = 600
