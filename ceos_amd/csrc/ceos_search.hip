@@ -738,10 +738,10 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
         };
         // Single pass with few candidates taken (ce well below the distinct ids): a lower
         // bound L of the ce-th largest sum is read off a strided sample of kSNT tile slots
-        // (the sample value of about twice the expected rank, plus 4); every slot whose
+        // (the sample value at 1.5 times the expected rank, plus 4); every slot whose
         // sum is at least L goes to scratch as a (sum, id) key in one pass over the tile.
         // When at least ce keys pass, the top-ce keys are all among them and are selected
-        // exactly; otherwise (a 5-sigma event) the histogram select below runs. The tile pass
+        // exactly; otherwise (a rare event) the histogram select below runs. The tile pass
         // writes (sum key, slot) pairs only; the ids are then fetched for the pairs kept, all
         // loads in flight together, and the keys go to the tile (when they fit) or stay in
         // scratch. Returns the number of keys, or -1.
@@ -760,7 +760,7 @@ __global__ __launch_bounds__(kSNT) void k_search_fused(IndexDims dm, const int32
             if (ln == 0) atomicAdd(reinterpret_cast<uint32_t*>(&misc[14]), occ);
             __syncthreads();
             const int nso = misc[14];
-            const int rs = (int)((2ll * ce * nso + ncand - 1) / max(ncand, 1)) + 4;
+            const int rs = (int)((3ll * ce * nso + 2ll * ncand - 1) / (2ll * max(ncand, 1))) + 4;
             if (2 * rs > nso) return -1;  // uniform: ce is a large share of the ids
             uint64_t prefix;
             int shift;
