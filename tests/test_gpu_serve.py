@@ -58,6 +58,31 @@ def test_serve_large_batches(CEOs, index):
         assert np.array_equal(bits(ips), bits(p)), i
 
 
+def test_serve_backlog_shared_among_free_slots(CEOs, index):
+    """4,000 requests arriving at once with four free slots of 4,096: the backlog is
+    shared among the slots (a quarter each, at least 256) instead of one slot taking
+    all of it while the device waits for its copy."""
+    X, g = index
+    n = 4000
+    Q = _rand(n, 64, 63)
+    k = np.full(n, 10, np.int32)
+    s = np.full(n, 20, np.int32)
+    c = np.full(n, 200, np.int32)
+    srv = CEOs.Server(g, max_batch=4096, k_max=10, s_max=20, c_max=200, inflight=4)
+    t0 = srv.submit(Q, k, s, c)
+    srv.drain()
+    trace = srv.batches()
+    got = [srv.result(t0 + i) for i in (0, n // 2, n - 1)]
+    srv.close()
+    assert trace[:, 4].sum() == n
+    assert trace[0, 4] <= -(-n // 4)  # the first batch took its share, not the backlog
+    assert len(trace) >= 4
+    for i, (ids, ips, _) in zip((0, n // 2, n - 1), got):
+        a, p = _alone(g, Q[i], 10, 20, 200)
+        assert np.array_equal(ids, a), i
+        assert np.array_equal(bits(ips), bits(p)), i
+
+
 def test_serve_open_loop_each_result_equals_alone(CEOs, index):
     X, g = index
     n = 600
