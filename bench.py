@@ -255,7 +255,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); it runs on a single GPU (the multi-GPU sharding is a design, not code)",
+        print(f"bench.py measures one GPU (got --gpus {args.gpus}): the multi-GPU sharding is a design, not code",
               file=sys.stderr)
         sys.exit(2)
 
