@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY. This file is a plain C++17/OpenMP restatement of the
 // reference algorithms, written from reading them. Only the test suite, the smoke
-// check and the CPU-baseline leg of bench.py load it, always as the checker; the
+// check and the CPU-baseline leg of bench.py load it, always as the reference answer; the
 // product (ceos_amd/) never links or calls it.
 //
 // Every function names the reference file:line it restates (paths are relative to

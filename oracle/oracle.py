@@ -1,7 +1,7 @@
 """ctypes wrapper over the CPU oracle (libceos_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
-CPU-baseline leg of bench.py, always as the checker. The product never imports it.
+CPU-baseline leg of bench.py, always as the reference answer. The product never imports it.
 
 The classes mirror the reference's Python surface (python/python_wrapper.cpp:13-75):
 `OracleCEOs` for class `CEOs`, `OracleStream` for class `streamCEOs`.

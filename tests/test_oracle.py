@@ -1,4 +1,4 @@
-"""CPU tests that pin the oracle before it is trusted as the checker.
+"""CPU tests that pin the oracle before it is trusted as the reference answer.
 
 Each oracle function is checked against something other than its own formula:
 the reference's own FFHT (compiled from the reference sources, or its committed
